@@ -1,0 +1,52 @@
+"""Shared test helpers: small random states and Philox-derived uniforms."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle
+
+RAND_MAX = 2 ** 31 - 1
+
+
+def random_geno(rng, N, L, A_max, miss=0.05, biallelic=False):
+    """Genotypes [N][2][L] int8 with recoded alleles and -9 for missing copies.
+
+    nall[l] = number of distinct alleles seen at l (first-appearance coding is
+    not needed here: any value < nall[l] is a valid recoded allele)."""
+    if biallelic:
+        nall = np.full(L, 2, np.int32)
+    else:
+        nall = rng.integers(1, A_max + 1, size=L).astype(np.int32)
+    geno = np.empty((N, 2, L), np.int8)
+    for l in range(L):
+        geno[:, :, l] = rng.integers(0, nall[l], size=(N, 2))
+    geno[rng.random((N, 2, L)) < miss] = -9
+    return geno, nall
+
+
+def random_q(rng, N, K, a=1.0):
+    return rng.dirichlet(np.full(K, a), size=N)
+
+
+def random_p(rng, L, K, A, nall, a=1.0):
+    P = np.zeros((L, K, A))
+    for l in range(L):
+        if nall[l] > 0:
+            P[l, :, : nall[l]] = rng.dirichlet(np.full(nall[l], a), size=K)
+    return P
+
+
+def z_uniforms(seed, chain, rep, N, L):
+    """randomArr for the reference UpdateZ: [(ind*L + loc)*2 + line] (UpdateZ.c:37)."""
+    r = np.zeros(N * L * 2)
+    for i in range(N):
+        for l in range(L):
+            w = oracle.philox_block(seed, chain, rep, l, i >> 1, oracle.PH_Z << 24)
+            for c in range(2):
+                r[(i * L + l) * 2 + c] = w[2 * (i & 1) + c] * 2.0 ** -32
+    return r
+
+
+def geno_ref_layout(geno):
+    """[N][2][L] int8 -> int32 in the reference GenPos layout (structure.h:22)."""
+    return np.ascontiguousarray(geno.astype(np.int32))
