@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark: admixture-MCMC sweeps/s on one MI355X at N=2000, L=100000, K=5.
+
+A step is one full sweep of the reference hot loop (structure.c:497-638) on
+synthetic Balding-Nichols SNP data (K_true=5, F in [0.01, 0.1], Q ~ Dir(0.3),
+1% missing) under the F model with alpha and lambda inference and
+METROFREQ=10: UpdateP, UpdateQ (Gibbs, Metropolis every 10th sweep), the fused
+UpdateZ, alpha, lambda, epsilon, Fst, and the post-burn-in data collection with
+lnP(D).  W warm-up sweeps are burn-in; the K timed sweeps are all post-burn-in
+(the costlier half of a run).  Genotypes, Q and P are resident in HBM before
+timing starts.
+
+Prints one JSON line (see the keys below); rank-0 only, one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def z_bytes_per_launch(N, L, K, A):
+    """Algorithmic HBM bytes of one fused UpdateZ launch: 2-bit genotypes
+    (N*L/4), P read (L*K*A doubles), allele counts written (L*K*A int32), Q read
+    (N*K doubles), ancestry counts written (N*K int32)."""
+    return N * L / 4 + L * K * A * 8 + L * K * A * 4 + N * K * 8 + N * K * 4
+
+
+def sweep_bytes(N, L, K, A):
+    """Algorithmic bytes per sweep: the UpdateZ traffic plus UpdateP's count read
+    and P write and one pass over epsilon."""
+    return z_bytes_per_launch(N, L, K, A) + L * K * A * (4 + 8) + L * A * 8
+
+
+def cpu_baseline(N, L, K, sample_loci, sweeps, seed):
+    import numpy as np
+
+    import oracle
+    import structure_amd as sa
+    d = sa.synth_snp(N, sample_loci, Ktrue=5, flo=0.01, fhi=0.1, qalpha=0.3, miss=0.01,
+                     seed=seed, want_geno=True)
+    m = oracle.make_model(N, sample_loci, K, 2, freqscorr=1, inferalpha=1, inferlambda=1,
+                          metrofreq=10, burnin=0, seed=seed, chain=0)
+    ch = oracle.Chain(m, d["geno"], d["nall"])
+    ch.init()
+    ch.sweep(0)   # untimed first sweep
+    t0 = time.perf_counter()
+    for r in range(1, 1 + sweeps):   # reps 1..10 include one Metropolis sweep (rep 10)
+        ch.sweep(r)
+    dt = time.perf_counter() - t0
+    del np
+    per_sweep_sample = dt / sweeps
+    # every phase of a sweep is linear in L at fixed N and K
+    per_sweep_full = per_sweep_sample * (L / sample_loci)
+    return {"value": 1.0 / per_sweep_full, "unit": "sweeps/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (FP64 C restatement, -O2) on N={N}, L={sample_loci} "
+                       f"(1/{L // sample_loci} of the loci), {sweeps} post-burn-in sweeps in "
+                       f"{dt:.1f} s, scaled linearly to L={L}")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--N", type=int, default=2000)
+    ap.add_argument("--L", type=int, default=100000)
+    ap.add_argument("--K", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-sample-loci", type=int, default=10000)
+    ap.add_argument("--cpu-sweeps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import structure_amd as sa
+    N, L, K = args.N, args.L, args.K
+    t0 = time.perf_counter()
+    d = sa.synth_snp(N, L, Ktrue=5, flo=0.01, fhi=0.1, qalpha=0.3, miss=0.01, seed=args.seed)
+    print(f"bench: synthetic data N={N} L={L} in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    model = sa.make_model(N, L, K, 2, freqscorr=1, inferalpha=1, inferlambda=1, metrofreq=10,
+                          burnin=args.warmup, numreps=args.steps, seed=args.seed, chain=0)
+    ch = sa.Chain(model, sa.GENO_SNP2, d["words"], d["nall"])
+    ch.init()
+    ch.run(0, args.warmup)
+    ch.synchronize()
+    print(f"bench: {args.warmup} warm-up sweeps done", file=sys.stderr, flush=True)
+
+    ch.set_timing(True)
+    ch.get_timing()   # reset
+    ch.synchronize()
+    t0 = time.perf_counter()
+    ch.run(args.warmup, args.steps)
+    ch.synchronize()
+    dt = time.perf_counter() - t0
+    timing = ch.get_timing()
+    ch.set_timing(False)
+    like = ch.get(sa.LIKE)
+    ch.close()
+
+    sweeps_per_s = args.steps / dt
+    zms, zn = timing["Z"]
+    z_avg_s = zms / max(zn, 1) / 1e3
+    zb = z_bytes_per_launch(N, L, K, 2)
+    achieved = zb / z_avg_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "updatez_c2_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as f:
+                traffic = json.load(f).get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    result = {
+        "metric": "sweeps/s at N=2k,L=100k,K=5 (%HBM peak); batched vs plain chain-sweeps/s",
+        "value": round(sweeps_per_s, 3),
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic (Balding-Nichols SNP, seed {args.seed}); random initial state",
+        "config": {"workload": (f"config 2: N={N} L={L} K={K} biallelic 2-bit, F model + "
+                                f"alpha + lambda, METROFREQ=10, post-burn-in sweeps with lnP(D)"),
+                   "N": N, "L": L, "K": K, "genotype_format": "SNP2"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
+                     "traffic": traffic, "kernel": "k_update_z_snp2 (fused UpdateZ + counts + lnP)",
+                     "algorithmic_bytes_per_launch": zb, "avg_launch_ms": round(z_avg_s * 1e3, 4)},
+        "site_updates_per_s": round(N * L * K * sweeps_per_s, 1),
+        "sweep_hbm_frac": round(sweep_bytes(N, L, K, 2) * sweeps_per_s / 1e9 / PEAK_HBM_GBS, 5),
+        "phase_ms_per_sweep": {p: round(v[0] / max(args.steps, 1), 4) for p, v in timing.items()},
+        "lnP_last": like[0],
+    }
+    if not args.no_cpu:
+        print("bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(N, L, K, args.cpu_sample_loci, args.cpu_sweeps,
+                                              args.seed)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

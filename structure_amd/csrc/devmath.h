@@ -1,0 +1,152 @@
+// Device-side random streams and special functions for the admixture sweep.
+//
+// Random stream contract (identical on every kernel): key = (seed, chain),
+// counter = (c0, c1, c2, (phase << 24) | attempt).  Phases and index
+// assignment are listed in sgpu_internal.h.  Philox4x32-10 follows Salmon et
+// al., SC'11; it replaces the reference's MWC64X device streams
+// (Kernels/randGen.cl, Kernels/mwc64x) and its host rand()/randomArr
+// (ran.c:128-161, randGen.c:23-36).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sgpu {
+
+struct u32x4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                               uint32_t c3, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint32_t lo0 = 0xD2511F53u * c0;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return u32x4{c0, c1, c2, c3};
+}
+
+__device__ __forceinline__ uint32_t word_of(const u32x4 &w, int j)
+{
+    return j == 0 ? w.x : (j == 1 ? w.y : (j == 2 ? w.z : w.w));
+}
+
+// uniform in [0,1), exact: categorical draws, RandomInteger, RandomReal (ran.c:128-147)
+__device__ __forceinline__ double uz(uint32_t w) { return (double)w * 0x1p-32; }
+// uniform in (0,1), exact: rnd() semantics (ran.c:152-161)
+__device__ __forceinline__ double uo(uint32_t w) { return ((double)w + 0.5) * 0x1p-32; }
+
+__device__ __forceinline__ double box_muller(uint32_t w0, uint32_t w1)
+{
+    double r = sqrt(-2.0 * log(uo(w0)));
+    return r * cos(6.283185307179586 * uo(w1));
+}
+
+__device__ __forceinline__ double normal_draw(uint32_t seed, uint32_t chain, uint32_t c0,
+                                              uint32_t c1, uint32_t c2, uint32_t phase)
+{
+    u32x4 w = philox4x32_10(c0, c1, c2, phase << 24, seed, chain);
+    return box_muller(w.x, w.y);
+}
+
+constexpr uint32_t kGammaMaxAttempts = 256;
+
+// Gamma(a, 1), Marsaglia & Tsang (ACM TOMS 26(3), 2000); a < 1 through
+// Gamma(a+1) * U^(1/a).  One Philox block per attempt: words 0,1 -> normal,
+// 2 -> acceptance uniform, 3 -> boost uniform.  FP64 throughout: with
+// shapes in the thousands (allele / ancestry counts) the acceptance test
+// d*(1 - v + ln v) cancels catastrophically in FP32.
+__device__ __forceinline__ double gamma_draw(double a, uint32_t seed, uint32_t chain,
+                                             uint32_t c0, uint32_t c1, uint32_t c2,
+                                             uint32_t phase)
+{
+    if (!(a > 0.0)) {
+        return 0.0;
+    }
+    const double aa = a < 1.0 ? a + 1.0 : a;
+    const double d = aa - 1.0 / 3.0;
+    const double c = 1.0 / sqrt(9.0 * d);
+    for (uint32_t j = 0; j < kGammaMaxAttempts; j++) {
+        u32x4 w = philox4x32_10(c0, c1, c2, (phase << 24) | j, seed, chain);
+        const double x = box_muller(w.x, w.y);
+        double v = 1.0 + c * x;
+        if (v <= 0.0) {
+            continue;
+        }
+        v = v * v * v;
+        const double u = uo(w.z);
+        const double x2 = x * x;
+        if (u < 1.0 - 0.0331 * (x2 * x2) || log(u) < 0.5 * x2 + d * (1.0 - v + log(v))) {
+            double g = d * v;
+            if (a < 1.0) {
+                g = g * exp(log(uo(w.w)) / a);
+            }
+            return g;
+        }
+    }
+    return d;
+}
+
+// ln Gamma(z), z > 0: Lanczos g = 7, n = 9 with the published coefficients,
+// the approximation the reference uses on the host (mymath.c:382-429).
+__device__ __forceinline__ double lgamma_lanczos(double z)
+{
+    const double c1 = 676.5203681218835, c2 = -1259.139216722289, c3 = 771.3234287757674,
+                 c4 = -176.6150291498386, c5 = 12.50734324009056, c6 = -0.1385710331296526,
+                 c7 = 9.934937113930748e-6, c8 = 1.659470187408462e-7;
+    // denominators z+7 .. z+0 by repeated subtraction, as the host routine does
+    double s = 0.0, den = z + 7.0;
+    s += c8 / den; den -= 1.0;
+    s += c7 / den; den -= 1.0;
+    s += c6 / den; den -= 1.0;
+    s += c5 / den; den -= 1.0;
+    s += c4 / den; den -= 1.0;
+    s += c3 / den; den -= 1.0;
+    s += c2 / den; den -= 1.0;
+    s += c1 / den;
+    s += 0.9999999999995183;
+    return log(s) + 0.9189385332046727 - (z + 6.5) + (z - 0.5) * log(z + 6.5);
+}
+
+// Wave-level (64 lanes) and block-level FP64 sums with a fixed reduction tree.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_down(v, off, 64);
+    }
+    return v;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *lds /* NT/64 */)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        lds[wid] = v;
+    }
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < NT / 64; i++) {
+            t += lds[i];
+        }
+    }
+    __syncthreads();
+    return t;   // valid in thread 0
+}
+
+}  // namespace sgpu

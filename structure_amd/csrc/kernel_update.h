@@ -1,0 +1,739 @@
+// Parameter updates of the admixture sweep: UpdateP, UpdateQ (Gibbs and
+// Metropolis), alpha, lambda, epsilon, Fst and the data-collection tail.
+// Reductions feeding a Metropolis decision are FP64 with a fixed tree
+// (block partials in block order, then one block), so results do not depend
+// on scheduling.
+#pragma once
+
+#include "devmath.h"
+#include "sgpu_internal.h"
+
+namespace sgpu {
+
+constexpr int kNT = 256;   // threads per block for the update kernels
+
+struct ModelArgs {
+    int N, L, K, A;
+    int freqscorr, onefst, popalphas, unifprioralpha, popspecificlambda;
+    double alphapropsd, alphamax, alphapriora, alphapriorb, fpriormean, fpriorsd;
+    uint32_t seed, chain;
+};
+
+// ---------------------------------------------------------------------------
+// UpdateP (UpdateP.c:106-199): one thread per (locus, pop).  Dirichlet
+// parameters from the allele counts of the last UpdateZ (summed over the
+// individual-tile partials), draw via LogRDirichletDisc's rule
+// (ran.c:1247-1272) including the sum < UNDERFLO uniform fallback.
+// Fused: PSum += P (output.c:673-677) when collecting, and the per-pop sums of
+// log P that UpdatePopLambda / UpdateLambda need without the F model.
+// ---------------------------------------------------------------------------
+template <int ASTORE>
+__global__ __launch_bounds__(kNT) void k_update_p(ModelArgs m, const int32_t *__restrict__ nall,
+                                                   const int32_t *__restrict__ nApart, int n_it,
+                                                   const double *__restrict__ eps,
+                                                   const double *__restrict__ F,
+                                                   const double *__restrict__ lambda,
+                                                   double *__restrict__ P,
+                                                   double *__restrict__ PSum, int collect,
+                                                   uint32_t rep)
+{
+    const int idx = blockIdx.x * kNT + threadIdx.x;
+    if (idx >= m.L * m.K) {
+        return;
+    }
+    const int l = idx / m.K, k = idx % m.K;
+    const int na = nall[l];
+    const size_t base = ((size_t)l * m.K + k) * m.A;
+    const size_t stride = (size_t)m.L * m.K * m.A;
+    double g[ASTORE > 0 ? ASTORE : 1];
+    double sum = 0.0;
+    const double Fk = F[k];
+    for (int aa = 0; aa < na; aa++) {
+        int n = 0;
+        for (int it = 0; it < n_it; it++) {
+            n += nApart[it * stride + base + aa];
+        }
+        double par;
+        if (m.freqscorr) {
+            par = eps[(size_t)l * m.A + aa] * (1.0 - Fk) / Fk + (double)n;
+        } else {
+            par = lambda[k] + (double)n;
+        }
+        const double x = gamma_draw(par, m.seed, m.chain, rep, (uint32_t)l,
+                                    ((uint32_t)k << 16) | (uint32_t)aa, PH_P);
+        if (ASTORE > 0) {
+            g[aa] = x;
+        }
+        sum += x;
+    }
+    const bool uniform = sum < kUnderflo;
+    for (int aa = 0; aa < na; aa++) {
+        double x;
+        if (ASTORE > 0) {
+            x = g[aa];
+        } else {
+            // recompute the identical draw (same counter) instead of storing it
+            int n = 0;
+            for (int it = 0; it < n_it; it++) {
+                n += nApart[it * stride + base + aa];
+            }
+            const double par = m.freqscorr ? eps[(size_t)l * m.A + aa] * (1.0 - Fk) / Fk + (double)n
+                                           : lambda[k] + (double)n;
+            x = gamma_draw(par, m.seed, m.chain, rep, (uint32_t)l,
+                           ((uint32_t)k << 16) | (uint32_t)aa, PH_P);
+        }
+        const double p = uniform ? 1.0 / (double)na : x / sum;
+        P[base + aa] = p;
+        if (collect) {
+            PSum[base + aa] += p;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dirichlet(alpha + counts) for one individual into q[] (UpdateQAdmixture,
+// UpdateQ.c:387-424 / RDirichlet ran.c:348-359), or the Metropolis proposal
+// Dirichlet(alpha) (RDirichletDisc, ran.c:1198-1210) when counts is null.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dirichlet_q(const ModelArgs &m, const double *alpha,
+                                            const int32_t *cnt, uint32_t rep, int i,
+                                            uint32_t phase, double *q)
+{
+    double sum = 0.0;
+    for (int k = 0; k < m.K; k++) {
+        const double par = cnt ? alpha[k] + (double)cnt[k] : alpha[k];
+        q[k] = gamma_draw(par, m.seed, m.chain, rep, (uint32_t)i, (uint32_t)k, phase);
+        sum += q[k];
+    }
+    for (int k = 0; k < m.K; k++) {
+        q[k] /= sum;
+    }
+}
+
+// Store Q[i], add QSum, and accumulate the log-Q statistics the alpha update
+// reads: per-pop sum of log Q (LogProbQonepop, UpdateAlpha.c:102-111) and the
+// total over the clamped Q (LogProbQ, UpdateAlpha.c:157-180).
+__device__ __forceinline__ void finish_q(const ModelArgs &m, int i, bool active, const double *q,
+                                         double *Q, double *QSum, int collect, double *logq_part,
+                                         double *s_red)
+{
+    double tot = 0.0;
+    for (int k = 0; k < m.K; k++) {
+        if (active) {
+            Q[(size_t)i * m.K + k] = q[k];
+            if (collect) {
+                QSum[(size_t)i * m.K + k] += q[k];
+            }
+            tot += log(q[k] > kSqrtUnder ? q[k] : kSqrtUnder);
+        }
+    }
+    for (int k = 0; k <= m.K; k++) {
+        double v = 0.0;
+        if (active) {
+            v = (k < m.K) ? log(q[k]) : tot;
+        }
+        const double s = block_sum<kNT>(v, s_red);
+        if (threadIdx.x == 0) {
+            logq_part[(size_t)blockIdx.x * (m.K + 1) + k] = s;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kNT) void k_update_q_gibbs(ModelArgs m, const int32_t *__restrict__ nQpart,
+                                                         int n_lt, const double *__restrict__ alpha,
+                                                         double *__restrict__ Q,
+                                                         double *__restrict__ QSum, int collect,
+                                                         double *__restrict__ logq_part,
+                                                         uint32_t rep)
+{
+    __shared__ double s_red[kNT / 64];
+    const int i = blockIdx.x * kNT + threadIdx.x;
+    const bool active = i < m.N;
+    double q[kMaxK];
+    if (active) {
+        int32_t cnt[kMaxK];
+        for (int k = 0; k < m.K; k++) {
+            int n = 0;
+            for (int t = 0; t < n_lt; t++) {
+                n += nQpart[((size_t)t * m.N + i) * m.K + k];
+            }
+            cnt[k] = n;
+        }
+        dirichlet_q(m, alpha, cnt, rep, i, PH_Q, q);
+    }
+    finish_q(m, i, active, q, Q, QSum, collect, logq_part, s_red);
+}
+
+// Log-Q statistics of the current Q (after a host write of Q).
+__global__ __launch_bounds__(kNT) void k_logq_stats(ModelArgs m, const double *__restrict__ Q,
+                                                     double *__restrict__ logq_part)
+{
+    __shared__ double s_red[kNT / 64];
+    const int i = blockIdx.x * kNT + threadIdx.x;
+    const bool active = i < m.N;
+    double q[kMaxK];
+    for (int k = 0; k < m.K; k++) {
+        q[k] = active ? Q[(size_t)i * m.K + k] : 1.0;
+    }
+    double tot = 0.0;
+    for (int k = 0; k < m.K; k++) {
+        tot += active ? log(q[k] > kSqrtUnder ? q[k] : kSqrtUnder) : 0.0;
+    }
+    for (int k = 0; k <= m.K; k++) {
+        const double v = active ? ((k < m.K) ? log(q[k]) : tot) : 0.0;
+        const double s = block_sum<kNT>(v, s_red);
+        if (threadIdx.x == 0) {
+            logq_part[(size_t)blockIdx.x * (m.K + 1) + k] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// UpdateQMetro (UpdateQ.c:199-307): proposal from the prior, accepted with
+// probability exp(sum over copies of log(TestQ.P) - log(Q.P))
+// (CalcLikeIndDiff, CalcLike.c:281-345).  Pass 1: one thread per
+// individual and a locus range per grid row; each thread regenerates its
+// proposal from the stream and accumulates both products in FP64 with
+// exponent bookkeeping.  Pass 2 sums the ranges and decides.
+// ---------------------------------------------------------------------------
+template <int FMT>
+__global__ __launch_bounds__(kNT) void k_metro_ratio(ModelArgs m, const uint32_t *__restrict__ geno,
+                                                      int Lpad, const double *__restrict__ Q,
+                                                      const double *__restrict__ P,
+                                                      const double *__restrict__ alpha,
+                                                      int loci_per_range,
+                                                      double *__restrict__ delta_part,
+                                                      uint32_t rep)
+{
+    const int i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= m.N) {
+        return;
+    }
+    const int K = m.K;
+    double tq[kMaxK], cq[kMaxK];
+    dirichlet_q(m, alpha, nullptr, rep, i, PH_QMETRO, tq);
+    for (int k = 0; k < K; k++) {
+        cq[k] = Q[(size_t)i * K + k];
+    }
+    const int l0 = blockIdx.y * loci_per_range;
+    const int l1 = min(l0 + loci_per_range, m.L);
+    double pp = 1.0, pm = 1.0;
+    int ep = 0, em = 0;
+    for (int l = l0; l < l1; l++) {
+        int xa, xb;
+        if (FMT == 1) {
+            const uint32_t word = geno[(size_t)(i >> 4) * Lpad + l];
+            const uint32_t code = (word >> (2 * (i & 15))) & 3u;
+            if (code == 3u) {
+                continue;
+            }
+            xa = code == 2u;
+            xb = code >= 1u;
+        } else {
+            const uint32_t word = geno[(size_t)(i >> 1) * Lpad + l];
+            xa = (int)((word >> (16 * (i & 1))) & 0xffu);
+            xb = (int)((word >> (16 * (i & 1) + 8)) & 0xffu);
+        }
+        const double *Pl = P + (size_t)l * K * m.A;
+        if (xa != 0xff) {
+            double tp = 0.0, tm = 0.0;
+            for (int k = 0; k < K; k++) {
+                tp += tq[k] * Pl[k * m.A + xa];
+                tm += cq[k] * Pl[k * m.A + xa];
+            }
+            pp *= tp;
+            pm *= tm;
+        }
+        if (xb != 0xff) {
+            double tp = 0.0, tm = 0.0;
+            for (int k = 0; k < K; k++) {
+                tp += tq[k] * Pl[k * m.A + xb];
+                tm += cq[k] * Pl[k * m.A + xb];
+            }
+            pp *= tp;
+            pm *= tm;
+        }
+        int e;
+        pp = frexp(pp, &e);
+        ep += e;
+        pm = frexp(pm, &e);
+        em += e;
+    }
+    delta_part[(size_t)blockIdx.y * m.N + i] = (log(pp) - log(pm)) + (double)(ep - em) * kLn2;
+}
+
+__global__ __launch_bounds__(kNT) void k_metro_accept(ModelArgs m, int n_ranges,
+                                                       const double *__restrict__ delta_part,
+                                                       const double *__restrict__ alpha,
+                                                       double *__restrict__ Q,
+                                                       double *__restrict__ QSum, int collect,
+                                                       double *__restrict__ logq_part,
+                                                       int32_t *__restrict__ accepted, uint32_t rep)
+{
+    __shared__ double s_red[kNT / 64];
+    const int i = blockIdx.x * kNT + threadIdx.x;
+    const bool active = i < m.N;
+    double q[kMaxK];
+    if (active) {
+        double delta = 0.0;
+        for (int r = 0; r < n_ranges; r++) {
+            delta += delta_part[(size_t)r * m.N + i];
+        }
+        const u32x4 w = philox4x32_10(rep, (uint32_t)i, 0xffffffffu, PH_QMETRO << 24, m.seed,
+                                      m.chain);
+        const bool acc = uo(w.x) < exp(delta);
+        if (acc) {
+            dirichlet_q(m, alpha, nullptr, rep, i, PH_QMETRO, q);
+        } else {
+            for (int k = 0; k < m.K; k++) {
+                q[k] = Q[(size_t)i * m.K + k];
+            }
+        }
+        if (accepted) {
+            accepted[i] = acc;
+        }
+    }
+    finish_q(m, i, active, q, Q, QSum, collect, logq_part, s_red);
+}
+
+// ---------------------------------------------------------------------------
+// UpdateAlpha (UpdateAlpha.c:189-253), one block: sums the log-Q statistics
+// and runs the pop loop in thread 0.  LogProbQ(a) = (a-1) * sum log(clamped Q)
+// + (lgamma(K a) - K lgamma(a)) * N; LogProbQonepop as in the reference.
+// As there, logprobdiff is zeroed once and only reset by the gamma prior.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNT) void k_update_alpha(ModelArgs m, const double *__restrict__ logq_part,
+                                                       int nparts, double *__restrict__ alpha,
+                                                       uint32_t rep)
+{
+    __shared__ double s_red[kNT / 64];
+    __shared__ double s_stat[kMaxK + 1];
+    for (int k = 0; k <= m.K; k++) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nparts; b += kNT) {
+            v += logq_part[(size_t)b * (m.K + 1) + k];
+        }
+        const double s = block_sum<kNT>(v, s_red);
+        if (threadIdx.x == 0) {
+            s_stat[k] = s;
+        }
+    }
+    if (threadIdx.x != 0) {
+        return;
+    }
+    const int K = m.K;
+    const double N = (double)m.N;
+    const int numalphas = m.popalphas ? K : 1;
+    double logprobdiff = 0.0;
+    for (int pop = 0; pop < numalphas; pop++) {
+        const u32x4 w = philox4x32_10(rep, (uint32_t)pop, 0u, PH_ALPHA << 24, m.seed, m.chain);
+        const double z = box_muller(w.x, w.y);
+        const double oldalpha = alpha[pop];
+        const double newalpha = oldalpha + m.alphapropsd * z;
+        if ((newalpha > 0) && ((newalpha < m.alphamax) || (!m.unifprioralpha))) {
+            if (!m.unifprioralpha) {
+                logprobdiff = (m.alphapriora - 1) * log(newalpha / oldalpha) +
+                              (oldalpha - newalpha) / m.alphapriorb;
+            }
+            if (m.popalphas) {
+                double sumalphas = 0.0;
+                for (int k = 0; k < K; k++) {
+                    sumalphas += alpha[k];
+                }
+                const double t = s_stat[pop];
+                logprobdiff -= (oldalpha - 1.0) * t +
+                               (lgamma_lanczos(sumalphas) - lgamma_lanczos(oldalpha)) * N;
+                sumalphas += newalpha - oldalpha;
+                logprobdiff += (newalpha - 1.0) * t +
+                               (lgamma_lanczos(sumalphas) - lgamma_lanczos(newalpha)) * N;
+            } else {
+                const double t = s_stat[K];
+                logprobdiff += (newalpha - 1.0) * t +
+                               (lgamma_lanczos(K * newalpha) - K * lgamma_lanczos(newalpha)) * N;
+                logprobdiff -= (oldalpha - 1.0) * t +
+                               (lgamma_lanczos(K * oldalpha) - K * lgamma_lanczos(oldalpha)) * N;
+            }
+            if (uo(w.z) < exp(logprobdiff)) {
+                alpha[pop] = newalpha;
+                if (!m.popalphas) {
+                    for (int k = 1; k < K; k++) {
+                        alpha[k] = newalpha;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lambda (UpdateLambda.c:8-102).  Pass 1: per-locus log sums (loci with more
+// than one allele): sum_a log eps (F model), sum_k sum_a log P, or per pop
+// sum_a log P (pop-specific).  Pass 2 (one block): the normaliser terms are
+// grouped by allele count (ahist = (value, number of loci) pairs).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNT) void k_lambda_partial(ModelArgs m, const int32_t *__restrict__ nall,
+                                                         const double *__restrict__ P,
+                                                         const double *__restrict__ eps,
+                                                         double *__restrict__ part)
+{
+    __shared__ double s_red[kNT / 64];
+    const int l = blockIdx.x * kNT + threadIdx.x;
+    const int na = (l < m.L) ? nall[l] : 0;
+    const int nsums = m.popspecificlambda ? m.K : 1;
+    for (int s = 0; s < nsums; s++) {
+        double v = 0.0;
+        if (na > 1) {
+            if (m.popspecificlambda) {
+                for (int aa = 0; aa < na; aa++) {
+                    v += log(P[((size_t)l * m.K + s) * m.A + aa]);
+                }
+            } else if (m.freqscorr) {
+                for (int aa = 0; aa < na; aa++) {
+                    v += log(eps[(size_t)l * m.A + aa]);
+                }
+            } else {
+                for (int k = 0; k < m.K; k++) {
+                    for (int aa = 0; aa < na; aa++) {
+                        v += log(P[((size_t)l * m.K + k) * m.A + aa]);
+                    }
+                }
+            }
+        }
+        const double t = block_sum<kNT>(v, s_red);
+        if (threadIdx.x == 0) {
+            part[(size_t)blockIdx.x * nsums + s] = t;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kNT) void k_lambda_finalize(ModelArgs m, const double *__restrict__ part,
+                                                          int nparts, const int32_t *__restrict__ ahist,
+                                                          int nhist, double *__restrict__ lambda,
+                                                          uint32_t rep)
+{
+    __shared__ double s_red[kNT / 64];
+    __shared__ double s_stat[kMaxK];
+    const int nsums = m.popspecificlambda ? m.K : 1;
+    for (int s = 0; s < nsums; s++) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nparts; b += kNT) {
+            v += part[(size_t)b * nsums + s];
+        }
+        const double t = block_sum<kNT>(v, s_red);
+        if (threadIdx.x == 0) {
+            s_stat[s] = t;
+        }
+    }
+    if (threadIdx.x != 0) {
+        return;
+    }
+    const double stoppop = m.popspecificlambda ? 1.0 : (m.freqscorr ? 1.0 : (double)m.K);
+    for (int s = 0; s < nsums; s++) {
+        const u32x4 w = philox4x32_10(rep, (uint32_t)s, 0u, PH_LAMBDA << 24, m.seed, m.chain);
+        const double z = box_muller(w.x, w.y);
+        const double old = lambda[s];
+        const double nw = old + kLambdaPropSd * z;
+        if ((nw > 0.0) && (nw < kLambdaMax)) {
+            double sum = 0.0;
+            for (int h = 0; h < nhist; h++) {
+                const double na = (double)ahist[2 * h];
+                const double cnt = (double)ahist[2 * h + 1];
+                double t = 0.0;
+                t += stoppop * lgamma_lanczos(na * nw);
+                t -= stoppop * lgamma_lanczos(na * old);
+                t += stoppop * na * lgamma_lanczos(old);
+                t -= stoppop * na * lgamma_lanczos(nw);
+                sum += cnt * t;
+            }
+            sum += (nw - old) * s_stat[s];
+            if (uo(w.z) < exp(sum)) {
+                if (m.popspecificlambda) {
+                    lambda[s] = nw;
+                } else {
+                    for (int k = 0; k < m.K; k++) {
+                        lambda[k] = nw;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Epsilon (NonIndependenceUpdateEpsilon, UpdateEpsilon.c:71-145): one thread
+// per locus.  Fused: SumEpsilon += eps (output.c:684-687) when collecting.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNT) void k_update_eps(ModelArgs m, const int32_t *__restrict__ nall,
+                                                     const double *__restrict__ P,
+                                                     const double *__restrict__ F,
+                                                     const double *__restrict__ lambda,
+                                                     double *__restrict__ eps,
+                                                     double *__restrict__ epsSum, int collect,
+                                                     uint32_t rep)
+{
+    const int l = blockIdx.x * kNT + threadIdx.x;
+    if (l >= m.L) {
+        return;
+    }
+    const int na = nall[l];
+    double *e = eps + (size_t)l * m.A;
+    if (na > 1) {
+        const u32x4 w = philox4x32_10(rep, (uint32_t)l, 0u, PH_EPS << 24, m.seed, m.chain);
+        const int a1 = (int)(uz(w.x) * (double)na);
+        int a2 = (int)(uz(w.y) * (double)(na - 1));
+        if (a2 >= a1) {
+            a2 += 1;
+        }
+        const double invsqrtn = pow((double)m.N, -0.5);
+        const double d = 0.0 + uz(w.z) * (invsqrtn - 0.0);
+        const double e1 = e[a1], e2 = e[a2];
+        if ((e1 + d < 1.0) && (e2 - d > 0.0)) {
+            double sum = 0.0;
+            for (int k = 0; k < m.K; k++) {
+                const double frac = (1.0 - F[k]) / F[k];
+                sum += lgamma_lanczos(frac * e1);
+                sum += lgamma_lanczos(frac * e2);
+                sum -= lgamma_lanczos(frac * (e1 + d));
+                sum -= lgamma_lanczos(frac * (e2 - d));
+                sum += frac * d * log(P[((size_t)l * m.K + k) * m.A + a1]);
+                sum -= frac * d * log(P[((size_t)l * m.K + k) * m.A + a2]);
+            }
+            const double lam = lambda[0];
+            if (lam != 1.0) {
+                sum += log(pow((e1 + d) * (e2 - d) / (e1) / (e2), (double)lam - 1.0));
+            }
+            if (uo(w.w) < exp(sum)) {
+                e[a1] = e1 + d;
+                e[a2] = e2 - d;
+            }
+        }
+    }
+    if (collect) {
+        for (int aa = 0; aa < na; aa++) {
+            epsSum[(size_t)l * m.A + aa] += e[aa];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fst (UpdateFst.c:142-204).  Every thread regenerates the proposals from the
+// stream; pass 1 sums, per locus and proposal group, the allele terms of
+// FlikeFreqs(new) - FlikeFreqs(old) (UpdateFst.c:22-51); pass 2 adds the
+// prior ratio (FPriorDiff, :10-18) and the per-locus lgamma(frac) terms and
+// decides.  ONEFST: one proposal, likelihood summed over all pops.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double fst_proposal(const ModelArgs &m, double oldf, int pop1,
+                                               uint32_t rep, uint32_t *uw)
+{
+    const u32x4 w = philox4x32_10(rep, (uint32_t)pop1, 0u, PH_FST << 24, m.seed, m.chain);
+    if (uw) {
+        *uw = w.z;
+    }
+    return oldf + m.fpriorsd * box_muller(w.x, w.y);
+}
+
+__global__ __launch_bounds__(kNT) void k_fst_partial(ModelArgs m, const int32_t *__restrict__ nall,
+                                                      const double *__restrict__ eps,
+                                                      const double *__restrict__ P,
+                                                      const double *__restrict__ F,
+                                                      double *__restrict__ part, uint32_t rep)
+{
+    __shared__ double s_red[kNT / 64];
+    const int l = blockIdx.x * kNT + threadIdx.x;
+    const int na = (l < m.L) ? nall[l] : 0;
+    const int numpops1 = m.onefst ? 1 : m.K;
+    for (int p1 = 0; p1 < numpops1; p1++) {
+        const double oldf = F[p1];
+        const double newf = fst_proposal(m, oldf, p1, rep, nullptr);
+        double v = 0.0;
+        if (newf > 0.0 && newf < 1.0 && na > 0) {
+            const double nfr = (1.0 - newf) / newf;
+            const double ofr = (1.0 - oldf) / oldf;
+            const int p2end = m.onefst ? m.K : p1 + 1;
+            for (int p2 = p1; p2 < p2end; p2++) {
+                for (int aa = 0; aa < na; aa++) {
+                    const double ep = eps[(size_t)l * m.A + aa];
+                    const double lp = log(P[((size_t)l * m.K + p2) * m.A + aa]);
+                    v += (nfr * ep * lp - lgamma_lanczos(nfr * ep)) -
+                         (ofr * ep * lp - lgamma_lanczos(ofr * ep));
+                }
+            }
+        }
+        const double t = block_sum<kNT>(v, s_red);
+        if (threadIdx.x == 0) {
+            part[(size_t)blockIdx.x * numpops1 + p1] = t;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kNT) void k_fst_finalize(ModelArgs m, const double *__restrict__ part,
+                                                       int nparts, int nloci_nonzero,
+                                                       double *__restrict__ F, uint32_t rep)
+{
+    __shared__ double s_red[kNT / 64];
+    __shared__ double s_stat[kMaxK];
+    const int numpops1 = m.onefst ? 1 : m.K;
+    for (int p1 = 0; p1 < numpops1; p1++) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nparts; b += kNT) {
+            v += part[(size_t)b * numpops1 + p1];
+        }
+        const double t = block_sum<kNT>(v, s_red);
+        if (threadIdx.x == 0) {
+            s_stat[p1] = t;
+        }
+    }
+    if (threadIdx.x != 0) {
+        return;
+    }
+    const double mu = m.fpriormean, sd = m.fpriorsd;
+    for (int p1 = 0; p1 < numpops1; p1++) {
+        uint32_t uw;
+        const double oldf = F[p1];
+        const double newf = fst_proposal(m, oldf, p1, rep, &uw);
+        if (newf > 0.0 && newf < 1.0) {
+            double lpd = ((mu * mu / (sd * sd) - 1) * log(newf / oldf) +
+                          (oldf - newf) * mu / (sd * sd));
+            const double nfr = (1.0 - newf) / newf;
+            const double ofr = (1.0 - oldf) / oldf;
+            const int ngroup = m.onefst ? m.K : 1;
+            lpd += (double)ngroup * (double)nloci_nonzero *
+                   (lgamma_lanczos(nfr) - lgamma_lanczos(ofr));
+            lpd += s_stat[p1];
+            const bool acc = (lpd >= 0.0) || (uo(uw) < exp(lpd));
+            if (acc) {
+                const int p2end = m.onefst ? m.K : p1 + 1;
+                for (int p2 = p1; p2 < p2end; p2++) {
+                    F[p2] = newf;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Data-collection tail (output.c:50-101, UpdateSums :654-707): scalar sums
+// and lnP(D) from the UpdateZ partials (one block, fixed order).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNT) void k_collect(ModelArgs m, const double *__restrict__ likepart,
+                                                  int nlike, int with_like,
+                                                  const double *__restrict__ alpha,
+                                                  const double *__restrict__ lambda,
+                                                  const double *__restrict__ F,
+                                                  double *__restrict__ alphaSum,
+                                                  double *__restrict__ lambdaSum,
+                                                  double *__restrict__ FstSum,
+                                                  double *__restrict__ like3)
+{
+    __shared__ double s_red[kNT / 64];
+    double v = 0.0;
+    if (with_like) {
+        for (int b = threadIdx.x; b < nlike; b += kNT) {
+            v += likepart[b];
+        }
+    }
+    const double like = block_sum<kNT>(v, s_red);
+    if (threadIdx.x != 0) {
+        return;
+    }
+    for (int k = 0; k < m.K; k++) {
+        lambdaSum[k] += lambda[k];
+        alphaSum[k] += alpha[k];
+        if (m.freqscorr) {
+            FstSum[k] += F[k];
+        }
+    }
+    if (with_like) {
+        like3[0] = like;
+        like3[1] += like;
+        like3[2] += like * like;
+    }
+}
+
+// lnP(D) of the current state without collecting (diagnostics).
+__global__ __launch_bounds__(kNT) void k_like_only(const double *__restrict__ likepart, int nlike,
+                                                    double *__restrict__ like3)
+{
+    __shared__ double s_red[kNT / 64];
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nlike; b += kNT) {
+        v += likepart[b];
+    }
+    const double like = block_sum<kNT>(v, s_red);
+    if (threadIdx.x == 0) {
+        like3[0] = like;
+    }
+}
+
+// Sum the per-tile count partials into full arrays (for state reads).
+__global__ void k_reduce_counts(const int32_t *__restrict__ part, int nparts, size_t n,
+                                int32_t *__restrict__ out)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) {
+        return;
+    }
+    int32_t s = 0;
+    for (int t = 0; t < nparts; t++) {
+        s += part[(size_t)t * n + j];
+    }
+    out[j] = s;
+}
+
+// Allele counts per locus for InitFreqPriors (Init.c:197-227), one thread per locus.
+__global__ void k_init_eps(ModelArgs m, int fmt, const uint32_t *__restrict__ geno, int Lpad,
+                           const int32_t *__restrict__ nall, double lambda0, double *__restrict__ eps)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= m.L) {
+        return;
+    }
+    const int na = nall[l];
+    double *e = eps + (size_t)l * m.A;
+    for (int aa = 0; aa < m.A; aa++) {
+        e[aa] = 0.0;
+    }
+    if (!m.freqscorr) {
+        for (int aa = 0; aa < na; aa++) {
+            e[aa] = lambda0;
+        }
+        return;
+    }
+    int total = 0;
+    // counts accumulate into e[] as integers first (exact in double)
+    if (fmt == 1) {
+        const int G = (m.N + 15) / 16;
+        for (int g = 0; g < G; g++) {
+            const uint32_t word = geno[(size_t)g * Lpad + l];
+            for (int j = 0; j < 16 && g * 16 + j < m.N; j++) {
+                const uint32_t code = (word >> (2 * j)) & 3u;
+                if (code != 3u) {
+                    total += 2;
+                    e[0] += (double)(2 - (int)code);
+                    if (na > 1) {
+                        e[1] += (double)code;
+                    }
+                }
+            }
+        }
+    } else {
+        const int G = (m.N + 1) / 2;
+        for (int g = 0; g < G; g++) {
+            const uint32_t word = geno[(size_t)g * Lpad + l];
+            for (int c = 0; c < 4 && g * 2 + (c >> 1) < m.N; c++) {
+                const int x = (int)((word >> (8 * c)) & 0xffu);
+                if (x != 0xff) {
+                    total++;
+                    if (x < na) {
+                        e[x] += 1.0;
+                    }
+                }
+            }
+        }
+    }
+    for (int aa = 0; aa < na; aa++) {
+        e[aa] = ((lambda0 + e[aa]) / ((double)na * lambda0 + (double)total));
+    }
+}
+
+}  // namespace sgpu

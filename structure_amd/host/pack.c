@@ -1,0 +1,98 @@
+/*
+ * Genotype packing for the sgpu C-ABI (host side).
+ *
+ * Input is the reference's recoded genotype array Geno[ind][line][loc]
+ * (structure.h:22 GenPos; alleles recoded 0..k-1 by first appearance,
+ * datain.c:824-904; -9 = missing).  Two device formats:
+ *   SGPU_GENO_SNP2  2 bits per (ind, locus), 16 individuals per uint32 word
+ *   SGPU_GENO_BYTE  1 byte per allele copy, 2 individuals per uint32 word
+ * Both are laid out [word row][Lpad] so that consecutive loci are
+ * consecutive words: the UpdateZ kernel maps one locus to one lane and
+ * streams a row with fully coalesced 256-byte wavefront loads.
+ */
+#include <stdint.h>
+#include <string.h>
+
+#include "../../include/sgpu.h"
+
+#define ZLOCI 256   /* loci per UpdateZ block; rows are padded to a multiple */
+
+static size_t lpad_of(int32_t L) { return ((size_t)L + ZLOCI - 1) / ZLOCI * ZLOCI; }
+
+size_t sgpu_packed_words(int32_t fmt, int32_t N, int32_t L)
+{
+    if (N < 1 || L < 1) {
+        return 0;
+    }
+    if (fmt == SGPU_GENO_SNP2) {
+        return ((size_t)N + 15) / 16 * lpad_of(L);
+    }
+    if (fmt == SGPU_GENO_BYTE) {
+        return ((size_t)N + 1) / 2 * lpad_of(L);
+    }
+    return 0;
+}
+
+int sgpu_pack_snp2(const int8_t *geno, int32_t N, int32_t L, uint32_t *out, size_t out_words)
+{
+    const size_t lp = lpad_of(L);
+    if (!geno || !out || out_words != sgpu_packed_words(SGPU_GENO_SNP2, N, L)) {
+        return SGPU_ERR_ARG;
+    }
+    memset(out, 0xff, out_words * sizeof(uint32_t));   /* padding = code 3 (missing) */
+    for (int32_t i = 0; i < N; i++) {
+        const int8_t *c0 = geno + (size_t)i * 2 * L;
+        const int8_t *c1 = c0 + L;
+        uint32_t *row = out + (size_t)(i >> 4) * lp;
+        const int sh = 2 * (i & 15);
+        for (int32_t l = 0; l < L; l++) {
+            const int a = c0[l], b = c1[l];
+            uint32_t code;
+            if (a == -9 && b == -9) {
+                code = 3u;
+            } else if (a == -9 || b == -9) {
+                return SGPU_ERR_UNSUPPORTED;   /* one copy missing */
+            } else if (a == 0 && b == 0) {
+                code = 0u;
+            } else if (a == 0 && b == 1) {
+                code = 1u;
+            } else if (a == 1 && b == 1) {
+                code = 2u;
+            } else {
+                return SGPU_ERR_UNSUPPORTED;   /* allele > 1 or heterozygote stored as (1,0) */
+            }
+            row[l] = (row[l] & ~(3u << sh)) | (code << sh);
+        }
+    }
+    return SGPU_OK;
+}
+
+int sgpu_pack_byte(const int8_t *geno, int32_t N, int32_t L, uint32_t *out, size_t out_words)
+{
+    const size_t lp = lpad_of(L);
+    if (!geno || !out || out_words != sgpu_packed_words(SGPU_GENO_BYTE, N, L)) {
+        return SGPU_ERR_ARG;
+    }
+    memset(out, 0xff, out_words * sizeof(uint32_t));
+    for (int32_t i = 0; i < N; i++) {
+        uint32_t *row = out + (size_t)(i >> 1) * lp;
+        const int sh = 16 * (i & 1);
+        for (int c = 0; c < 2; c++) {
+            const int8_t *g = geno + ((size_t)i * 2 + c) * L;
+            for (int32_t l = 0; l < L; l++) {
+                const int x = g[l];
+                uint32_t v;
+                if (x == -9) {
+                    v = 0xffu;
+                } else if (x >= 0 && x < 255) {
+                    v = (uint32_t)x;
+                } else {
+                    return SGPU_ERR_ARG;
+                }
+                const int s = sh + 8 * c;
+                row[l] = (row[l] & ~(0xffu << s)) | (v << s);
+            }
+        }
+    }
+    return SGPU_OK;
+}

@@ -1,0 +1,274 @@
+"""GPU parity: every HIP phase against the CPU oracle on identical state and
+the same Philox stream, through the C-ABI (libsgpu.so).
+
+Bars: Z, allele/ancestry counts and accept/reject decisions bit-exact; FP64
+draws (P, Q, alpha, lambda, eps, F) within 1e-12 relative (device and glibc
+transcendentals may differ in the last ulp); lnP(D) within 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import structure_amd as sa
+from helpers import random_geno, random_p, random_q
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def _pair(geno, nall, K, A, fmt=None, **kw):
+    N, _, L = geno.shape
+    om = oracle.make_model(N, L, K, A, **kw)
+    gm = sa.make_model(N, L, K, A, **kw)
+    och = oracle.Chain(om, geno, nall)
+    gch = sa.Chain.from_geno(gm, geno, nall, fmt)
+    return och, gch
+
+
+def _snp(N, L, K, seed, miss=0.02):
+    d = sa.synth_snp(N, L, Ktrue=K, qalpha=0.5, miss=miss, seed=seed, want_geno=True)
+    return d["geno"], d["nall"], 2
+
+
+def _close(a, b, rtol=RTOL):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    assert a.shape == b.shape
+    err = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+    assert np.all((a == b) | (err <= rtol)), f"max rel err {err.max():.3e}"
+
+
+def _state(gch, nall, rng, K, A, fq=1.0):
+    m = gch.model
+    Q = random_q(rng, m.N, K, a=fq)
+    P = random_p(rng, m.L, K, A, nall, a=1.0)
+    gch.set(sa.Q, Q)
+    gch.set(sa.P, P)
+    return Q, P
+
+
+CASES_Z = [("snp", 1), ("snp", 2), ("snp", 5), ("snp", 10), ("byte", 3), ("byte", 7),
+           ("byteraw", 4), ("byteraw", 12)]
+
+
+@pytest.mark.parametrize("kind,K", CASES_Z)
+def test_update_z_counts_and_like_bitexact(kind, K):
+    rng = np.random.default_rng(1000 + K)
+    if kind == "snp":
+        geno, nall, A = _snp(77, 600, min(K, 5), seed=K)
+        fmt = sa.GENO_SNP2
+    elif kind == "byte":
+        d = sa.synth_microsat(45, 333, amin=2, amax=9, Ktrue=3, seed=K)
+        geno, nall, A = d["geno"], d["nall"], d["A"]
+        fmt = sa.GENO_BYTE
+    else:   # per-copy missingness and heterozygotes in both orders
+        geno, nall = random_geno(rng, 39, 290, 6, miss=0.1)
+        A, fmt = 6, sa.GENO_BYTE
+    och, gch = _pair(geno, nall, K, A, fmt=fmt, seed=77, chain=5)
+    assert gch.fmt == fmt
+    gch.init()
+    Q, P = _state(gch, nall, rng, K, A, fq=0.6)
+    for rep in (3, 4):
+        z = gch.update_z(rep, like=True, want_z=True)
+        nA, nQ, Zo = och.update_z(Q, P, rep, want_z=True)
+        assert np.array_equal(z.astype(np.int32), Zo)
+        assert np.array_equal(gch.get(sa.NA), nA)
+        assert np.array_equal(gch.get(sa.NQ), nQ)
+        gch.collect(rep)
+        like = gch.get(sa.LIKE)[0]
+        _close(like, och.calc_like(Q, P))
+
+
+@pytest.mark.parametrize("kind", ["snp", "byte"])
+def test_init_state_matches_oracle(kind):
+    if kind == "snp":
+        geno, nall, A = _snp(50, 700, 3, seed=11)
+    else:
+        d = sa.synth_microsat(40, 120, amin=2, amax=8, Ktrue=3, seed=11)
+        geno, nall, A = d["geno"], d["nall"], d["A"]
+    for freqscorr in (1, 0):
+        och, gch = _pair(geno, nall, 4, A, seed=9, chain=1, freqscorr=freqscorr, lambda0=0.7)
+        gch.init()
+        och.init()
+        assert np.array_equal(gch.get(sa.NA), och.nA)
+        assert np.array_equal(gch.get(sa.NQ), och.nQ)
+        _close(gch.get(sa.EPS), och.eps, 1e-15)
+        assert np.array_equal(gch.get(sa.FST), och.F)
+        assert np.array_equal(gch.get(sa.Q), och.Q)
+        assert np.array_equal(gch.get(sa.ALPHA), och.alpha)
+
+
+@pytest.mark.parametrize("freqscorr", [1, 0])
+def test_update_p_matches_oracle(freqscorr):
+    rng = np.random.default_rng(7)
+    d = sa.synth_microsat(30, 200, amin=1, amax=12, Ktrue=3, seed=5)
+    geno, nall, A = d["geno"], d["nall"], d["A"]
+    K = 4
+    och, gch = _pair(geno, nall, K, A, seed=3, chain=2, freqscorr=freqscorr)
+    gch.init()
+    och.init()
+    Q, P = _state(gch, nall, rng, K, A)
+    nA, _ = och.update_z(Q, P, 1)
+    gch.set(sa.NA, nA)
+    eps = och.eps.copy()
+    F = rng.uniform(0.01, 0.4, K)
+    lam = rng.uniform(0.2, 2.0, K)
+    gch.set(sa.FST, F)
+    gch.set(sa.LAMBDA, lam)
+    for rep in (2, 3):
+        gch.update_p(rep, collect=True)
+        _close(gch.get(sa.P), och.update_p(nA, eps, F, lam, rep))
+    psum = gch.get(sa.PSUM)
+    _close(psum, och.update_p(nA, eps, F, lam, 2) + och.update_p(nA, eps, F, lam, 3))
+
+
+def test_update_q_gibbs_and_metro_match_oracle():
+    rng = np.random.default_rng(8)
+    geno, nall, A = _snp(70, 500, 3, seed=21)
+    K = 3
+    och, gch = _pair(geno, nall, K, A, seed=4, chain=0, metrofreq=10)
+    gch.init()
+    och.init()
+    Q, P = _state(gch, nall, rng, K, A)
+    _, nQ = och.update_z(Q, P, 1)
+    gch.set(sa.NQ, nQ)
+    alpha = np.array([0.4, 1.3, 0.8])
+    gch.set(sa.ALPHA, alpha)
+    gch.update_q(7)
+    _close(gch.get(sa.Q), och.update_q_gibbs(nQ, alpha, 7))
+    nacc = 0
+    for rep in (10, 20, 30):
+        Q0 = random_q(rng, 70, K, a=1.0)
+        gch.set(sa.Q, Q0)
+        gch.update_q(rep)
+        Qo, acc = och.update_q_metro(P, alpha, rep, Q0)
+        _close(gch.get(sa.Q), Qo)
+        nacc += acc.sum()
+    assert 0 < nacc < 3 * 70
+
+
+@pytest.mark.parametrize("popalphas,unif", [(0, 1), (1, 1), (0, 0)])
+def test_update_alpha_matches_oracle(popalphas, unif):
+    rng = np.random.default_rng(9)
+    geno, nall, A = _snp(40, 300, 3, seed=2)
+    K = 4
+    och, gch = _pair(geno, nall, K, A, popalphas=popalphas, unifprioralpha=unif,
+                     alphapropsd=0.2, alphamax=2.5, seed=12)
+    gch.init()
+    moved = 0
+    for rep in range(40):
+        Q = random_q(rng, 40, K, a=rng.uniform(0.3, 2))
+        alpha = rng.uniform(0.2, 2.0, K) if popalphas else np.full(K, rng.uniform(0.2, 2))
+        gch.set(sa.Q, Q)
+        gch.set(sa.ALPHA, alpha)
+        gch.update_alpha(rep)
+        ao = och.update_alpha(Q, alpha, rep)
+        _close(gch.get(sa.ALPHA), ao)
+        moved += int(not np.array_equal(ao, alpha))
+    assert 3 < moved < 40
+
+
+@pytest.mark.parametrize("freqscorr,popspec", [(1, 0), (0, 0), (0, 1)])
+def test_update_lambda_matches_oracle(freqscorr, popspec):
+    rng = np.random.default_rng(10)
+    d = sa.synth_microsat(30, 150, amin=1, amax=6, Ktrue=3, seed=9)
+    geno, nall, A = d["geno"], d["nall"], d["A"]
+    K = 3
+    och, gch = _pair(geno, nall, K, A, freqscorr=freqscorr, popspecificlambda=popspec,
+                     inferlambda=1, seed=13)
+    gch.init()
+    och.init()
+    moved = 0
+    for rep in range(30):
+        P = random_p(rng, 150, K, A, nall, a=rng.uniform(0.3, 3))
+        eps = np.zeros((150, A))
+        for l in range(150):
+            if nall[l]:
+                eps[l, :nall[l]] = rng.dirichlet(np.ones(nall[l]))
+        lam = rng.uniform(0.2, 3, K) if popspec else np.full(K, rng.uniform(0.2, 3))
+        gch.set(sa.P, P)
+        gch.set(sa.EPS, eps)
+        gch.set(sa.LAMBDA, lam)
+        gch.update_lambda(rep)
+        lo = och.update_lambda(P, eps, lam, rep)
+        _close(gch.get(sa.LAMBDA), lo)
+        moved += int(not np.array_equal(lo, lam))
+    assert 2 < moved < 30
+
+
+@pytest.mark.parametrize("lam0", [1.0, 0.6])
+def test_update_eps_matches_oracle(lam0):
+    rng = np.random.default_rng(11)
+    d = sa.synth_microsat(25, 400, amin=1, amax=7, Ktrue=3, seed=10)
+    geno, nall, A = d["geno"], d["nall"], d["A"]
+    K = 3
+    och, gch = _pair(geno, nall, K, A, seed=14)
+    gch.init()
+    och.init()
+    P = random_p(rng, 400, K, A, nall, a=2.0)
+    F = rng.uniform(0.02, 0.3, K)
+    eps = och.eps.copy()
+    gch.set(sa.P, P)
+    gch.set(sa.FST, F)
+    gch.set(sa.LAMBDA, np.full(K, lam0))
+    gch.set(sa.EPS, eps)
+    for rep in (1, 2, 3):
+        gch.update_eps(rep, collect=True)
+        eps = och.update_eps(P, F, lam0, eps, rep)
+        _close(gch.get(sa.EPS), eps)
+    assert not np.array_equal(eps, och.eps)
+
+
+@pytest.mark.parametrize("onefst", [0, 1])
+def test_update_fst_matches_oracle(onefst):
+    rng = np.random.default_rng(12)
+    d = sa.synth_microsat(25, 300, amin=1, amax=5, Ktrue=3, seed=12)
+    geno, nall, A = d["geno"], d["nall"], d["A"]
+    K = 3
+    och, gch = _pair(geno, nall, K, A, onefst=onefst, fpriormean=0.05, fpriorsd=0.05, seed=15)
+    gch.init()
+    och.init()
+    moved = 0
+    for rep in range(30):
+        P = random_p(rng, 300, K, A, nall, a=2.0)
+        eps = och.eps.copy()
+        F = np.full(K, rng.uniform(0.02, 0.2)) if onefst else rng.uniform(0.02, 0.2, K)
+        gch.set(sa.P, P)
+        gch.set(sa.EPS, eps)
+        gch.set(sa.FST, F)
+        gch.update_fst(rep)
+        fo = och.update_fst(eps, P, F, rep)
+        _close(gch.get(sa.FST), fo)
+        moved += int(not np.array_equal(fo, F))
+    assert 2 < moved < 30
+
+
+@pytest.mark.parametrize("kind", ["microsat", "snp_fmodel_lambda"])
+def test_chain_trajectory_matches_oracle(kind):
+    """Whole sweeps: the GPU chain and the oracle chain follow the same path."""
+    if kind == "microsat":   # config-1 shape, scaled down
+        d = sa.synth_microsat(60, 90, amin=2, amax=10, Ktrue=3, qalpha=0.5, miss=0.02, seed=1)
+        geno, nall, A = d["geno"], d["nall"], d["A"]
+        kw = dict(freqscorr=0, inferalpha=1, lambda0=1.0, burnin=8, metrofreq=10)
+        K = 3
+    else:                    # config-2 options, small shape
+        geno, nall, A = _snp(96, 400, 5, seed=3, miss=0.01)
+        kw = dict(freqscorr=1, inferalpha=1, inferlambda=1, burnin=8, metrofreq=10)
+        K = 5
+    och, gch = _pair(geno, nall, K, A, seed=2024, chain=1, **kw)
+    gch.init()
+    och.init()
+    for rep in range(31):
+        gch.sweep(rep)
+        och.sweep(rep)
+    for sel, ref in [(sa.Q, och.Q), (sa.P, och.P), (sa.EPS, och.eps), (sa.FST, och.F),
+                     (sa.ALPHA, och.alpha), (sa.LAMBDA, och.lambda_), (sa.QSUM, och.QSum),
+                     (sa.PSUM, och.PSum), (sa.ALPHASUM, och.alphaSum),
+                     (sa.FSTSUM, och.FstSum), (sa.EPSSUM, och.epsSum)]:
+        _close(gch.get(sel), ref, 1e-9)
+    assert np.array_equal(gch.get(sa.NA), och.nA)
+    assert np.array_equal(gch.get(sa.NQ), och.nQ)
+    like = gch.get(sa.LIKE)
+    _close(like, [och.like, och.sumlikes, och.sumsqlikes], 1e-9)
+    assert gch.get(sa.NSUM)[0] == och.state.nsum == 23

@@ -1,0 +1,143 @@
+"""CPU tests of the product's host side: C-ABI exports, packers, generators."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import structure_amd as sa
+from helpers import random_geno
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sgpu_\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header", ["sgpu.h", "sgpu_synth.h"])
+def test_library_exports_every_declared_symbol(header):
+    lib = ctypes.CDLL(sa.LIB_PATH)
+    names = _declared(header)
+    assert len(names) >= 2
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in include/{header} but not exported"
+
+
+def _unpack_snp2(words, N, L):
+    lpad = words.size // ((N + 15) // 16)
+    w = words.reshape(-1, lpad)[:, :L]
+    out = np.empty((N, 2, L), np.int8)
+    for i in range(N):
+        code = (w[i >> 4] >> np.uint32(2 * (i & 15))) & np.uint32(3)
+        out[i, 0] = np.where(code == 3, -9, code == 2)
+        out[i, 1] = np.where(code == 3, -9, code >= 1)
+    return out
+
+
+def _unpack_byte(words, N, L):
+    lpad = words.size // ((N + 1) // 2)
+    w = words.reshape(-1, lpad)[:, :L]
+    out = np.empty((N, 2, L), np.int8)
+    for i in range(N):
+        for c in range(2):
+            b = (w[i >> 1] >> np.uint32(16 * (i & 1) + 8 * c)) & np.uint32(0xff)
+            out[i, c] = np.where(b == 0xff, -9, b).astype(np.int8)
+    return out
+
+
+def test_pack_snp2_roundtrip_and_padding():
+    d = sa.synth_snp(37, 300, Ktrue=3, seed=4, miss=0.05, want_geno=True)
+    fmt, words = sa.pack(d["geno"])
+    assert fmt == sa.GENO_SNP2
+    assert np.array_equal(words, d["words"])
+    assert np.array_equal(_unpack_snp2(words, 37, 300), d["geno"])
+    lpad = words.size // 3
+    assert lpad % 256 == 0 and lpad >= 300
+    pad = words.reshape(3, lpad)[:, 300:]
+    assert np.all(pad == 0xffffffff)                       # padded loci are missing
+    assert np.all((words.reshape(3, lpad)[2, :300] >> np.uint32(2 * 5)) ==
+                  np.uint32(0xffffffff >> 10))              # individuals 37..47 missing
+
+
+def test_pack_byte_roundtrip_keeps_copy_order():
+    rng = np.random.default_rng(3)
+    geno, nall = random_geno(rng, 21, 77, 9, miss=0.15)
+    fmt, words = sa.pack(geno)
+    assert fmt == sa.GENO_BYTE   # half-missing sites and (1,0) hets force the byte path
+    assert np.array_equal(_unpack_byte(words, 21, 77), geno)
+
+
+@pytest.mark.parametrize("bad", [(1, 0), (0, -9), (-9, 1), (2, 2)])
+def test_pack_snp2_rejects_unrepresentable_sites(bad):
+    g = np.zeros((3, 2, 5), np.int8)
+    g[1, 0, 2], g[1, 1, 2] = bad
+    words = np.empty(sa.packed_words(sa.GENO_SNP2, 3, 5), np.uint32)
+    rc = sa.lib().sgpu_pack_snp2(g.ctypes.data_as(ctypes.c_void_p), 3, 5,
+                                 words.ctypes.data_as(ctypes.c_void_p), words.size)
+    assert rc == sa.ERR_UNSUPPORTED
+    with pytest.raises(sa.SgpuError):
+        sa.pack(g, sa.GENO_SNP2)
+
+
+def _check_first_appearance(geno, nall):
+    N, _, L = geno.shape
+    for l in range(L):
+        seen = []
+        for i in range(N):
+            for c in range(2):
+                x = int(geno[i, c, l])
+                if x == -9:
+                    continue
+                if x not in seen:
+                    assert x == len(seen), f"locus {l}: allele {x} appears before {len(seen)}"
+                    seen.append(x)
+        assert nall[l] == len(seen)
+
+
+def test_synth_snp_is_recoded_by_first_appearance_and_deterministic():
+    a = sa.synth_snp(50, 200, Ktrue=4, seed=9, want_geno=True, threads=1)
+    b = sa.synth_snp(50, 200, Ktrue=4, seed=9, want_geno=True, threads=7)
+    assert np.array_equal(a["words"], b["words"])
+    _check_first_appearance(a["geno"], a["nall"])
+    miss = (a["geno"][:, 0] == -9).mean()
+    assert 0.002 < miss < 0.03
+    het = a["geno"][:, 0] != a["geno"][:, 1]
+    assert np.all(a["geno"][:, 0][het] == 0)   # heterozygotes listed (0, 1)
+
+
+def test_synth_microsat_is_recoded_by_first_appearance():
+    d = sa.synth_microsat(40, 60, amin=2, amax=10, Ktrue=3, seed=5)
+    _check_first_appearance(d["geno"], d["nall"])
+    assert d["A"] == d["nall"].max() <= 10
+    assert d["nall"].min() >= 1
+
+
+def test_create_rejects_bad_shapes_before_touching_the_device():
+    d = sa.synth_snp(20, 30, Ktrue=2, seed=1)
+    with pytest.raises(sa.SgpuError) as e:
+        sa.Chain(sa.make_model(20, 30, 2, 3), sa.GENO_SNP2, d["words"], d["nall"])
+    assert e.value.code == sa.ERR_ARG            # SNP2 needs MAXALLELES == 2
+    with pytest.raises(sa.SgpuError) as e:
+        sa.Chain(sa.make_model(20, 30, 2, 2), sa.GENO_SNP2, d["words"][:-1], d["nall"])
+    assert e.value.code == sa.ERR_ARG
+    with pytest.raises(sa.SgpuError) as e:
+        sa.Chain(sa.make_model(20, 30, 17, 2), sa.GENO_SNP2, d["words"], d["nall"])
+    assert e.value.code == sa.ERR_UNSUPPORTED
+    bad = d["nall"].copy()
+    bad[bad == 2] = 1                            # allele 1 present but NumAlleles says 1
+    with pytest.raises(sa.SgpuError) as e:
+        sa.Chain(sa.make_model(20, 30, 2, 2), sa.GENO_SNP2, d["words"], bad)
+    assert e.value.code == sa.ERR_ARG
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
