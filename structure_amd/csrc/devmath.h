@@ -11,6 +11,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#ifndef SGPU_PHILOX_MULHI
+#define SGPU_PHILOX_MULHI 0
+#endif
+
 namespace sgpu {
 
 struct u32x4 {
@@ -20,6 +24,8 @@ struct u32x4 {
 __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
                                                uint32_t c3, uint32_t k0, uint32_t k1)
 {
+#if SGPU_PHILOX_MULHI
+    // separate low and high 32-bit products (v_mul_lo_u32 / v_mul_hi_u32)
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         const uint32_t lo0 = 0xD2511F53u * c0;
@@ -28,13 +34,29 @@ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
         const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
         const uint32_t n0 = hi1 ^ c1 ^ k0;
         const uint32_t n2 = hi0 ^ c3 ^ k1;
-        c0 = n0;
         c1 = lo1;
-        c2 = n2;
         c3 = lo0;
+        c0 = n0;
+        c2 = n2;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
+#else
+    // 32x32->64 products written as one 64-bit multiply each (v_mad_u64_u32)
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+#endif
     return u32x4{c0, c1, c2, c3};
 }
 

@@ -35,8 +35,9 @@ __global__ __launch_bounds__(kNT) void k_update_p(ModelArgs m, const int32_t *__
                                                    const double *__restrict__ lambda,
                                                    double *__restrict__ P,
                                                    double *__restrict__ PSum, int collect,
-                                                   uint32_t rep)
+                                                   const uint32_t *__restrict__ repp)
 {
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const int idx = blockIdx.x * kNT + threadIdx.x;
     if (idx >= m.L * m.K) {
         return;
@@ -113,6 +114,7 @@ __device__ __forceinline__ void dirichlet_q(const ModelArgs &m, const double *al
 // Store Q[i], add QSum, and accumulate the log-Q statistics the alpha update
 // reads: per-pop sum of log Q (LogProbQonepop, UpdateAlpha.c:102-111) and the
 // total over the clamped Q (LogProbQ, UpdateAlpha.c:157-180).
+template <int NT>
 __device__ __forceinline__ void finish_q(const ModelArgs &m, int i, bool active, const double *q,
                                          double *Q, double *QSum, int collect, double *logq_part,
                                          double *s_red)
@@ -132,36 +134,66 @@ __device__ __forceinline__ void finish_q(const ModelArgs &m, int i, bool active,
         if (active) {
             v = (k < m.K) ? log(q[k]) : tot;
         }
-        const double s = block_sum<kNT>(v, s_red);
+        const double s = block_sum<NT>(v, s_red);
         if (threadIdx.x == 0) {
             logq_part[(size_t)blockIdx.x * (m.K + 1) + k] = s;
         }
     }
 }
 
-__global__ __launch_bounds__(kNT) void k_update_q_gibbs(ModelArgs m, const int32_t *__restrict__ nQpart,
-                                                         int n_lt, const double *__restrict__ alpha,
-                                                         double *__restrict__ Q,
-                                                         double *__restrict__ QSum, int collect,
-                                                         double *__restrict__ logq_part,
-                                                         uint32_t rep)
+constexpr int kGibbsInds = 64;     // individuals per UpdateQ block
+constexpr int kRowSplit = 16;      // second-level partials of the ancestry counts
+
+// First level of the ancestry-count reduction: the n_lt locus-tile partials of
+// UpdateZ ([n_lt][N*K]) are folded into kRowSplit partials; grid (flat chunks,
+// kRowSplit), each thread sums every kRowSplit-th row of one flat index with
+// four independent accumulators.
+__global__ __launch_bounds__(kNT) void k_fold_rows(const int32_t *__restrict__ part, int nrows,
+                                                   size_t n, int32_t *__restrict__ out)
 {
-    __shared__ double s_red[kNT / 64];
-    const int i = blockIdx.x * kNT + threadIdx.x;
+    const size_t f = (size_t)blockIdx.x * kNT + threadIdx.x;
+    if (f >= n) {
+        return;
+    }
+    int32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int t = blockIdx.y;
+    for (; t + 3 * kRowSplit < nrows; t += 4 * kRowSplit) {
+        a0 += part[(size_t)t * n + f];
+        a1 += part[(size_t)(t + kRowSplit) * n + f];
+        a2 += part[(size_t)(t + 2 * kRowSplit) * n + f];
+        a3 += part[(size_t)(t + 3 * kRowSplit) * n + f];
+    }
+    for (; t < nrows; t += kRowSplit) {
+        a0 += part[(size_t)t * n + f];
+    }
+    out[(size_t)blockIdx.y * n + f] = (a0 + a1) + (a2 + a3);
+}
+
+// Gibbs update of Q from the folded counts ([kRowSplit][N][K]): one thread per
+// individual, 64 per block.
+__global__ __launch_bounds__(kGibbsInds) void k_update_q_gibbs(
+    ModelArgs m, const int32_t *__restrict__ nQfold, const double *__restrict__ alpha,
+    double *__restrict__ Q, double *__restrict__ QSum, int collect,
+    double *__restrict__ logq_part, const uint32_t *__restrict__ repp)
+{
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    __shared__ double s_red[1];
+    const int i = blockIdx.x * kGibbsInds + threadIdx.x;
     const bool active = i < m.N;
     double q[kMaxK];
     if (active) {
         int32_t cnt[kMaxK];
+        const size_t n = (size_t)m.N * m.K;
         for (int k = 0; k < m.K; k++) {
-            int n = 0;
-            for (int t = 0; t < n_lt; t++) {
-                n += nQpart[((size_t)t * m.N + i) * m.K + k];
+            int32_t c = 0;
+            for (int s2 = 0; s2 < kRowSplit; s2++) {
+                c += nQfold[s2 * n + (size_t)i * m.K + k];
             }
-            cnt[k] = n;
+            cnt[k] = c;
         }
         dirichlet_q(m, alpha, cnt, rep, i, PH_Q, q);
     }
-    finish_q(m, i, active, q, Q, QSum, collect, logq_part, s_red);
+    finish_q<kGibbsInds>(m, i, active, q, Q, QSum, collect, logq_part, s_red);
 }
 
 // Log-Q statistics of the current Q (after a host write of Q).
@@ -196,15 +228,101 @@ __global__ __launch_bounds__(kNT) void k_logq_stats(ModelArgs m, const double *_
 // proposal from the stream and accumulates both products in FP64 with
 // exponent bookkeeping.  Pass 2 sums the ranges and decides.
 // ---------------------------------------------------------------------------
-template <int FMT>
-__global__ __launch_bounds__(kNT) void k_metro_ratio(ModelArgs m, const uint32_t *__restrict__ geno,
-                                                      int Lpad, const double *__restrict__ Q,
-                                                      const double *__restrict__ P,
-                                                      const double *__restrict__ alpha,
-                                                      int loci_per_range,
-                                                      double *__restrict__ delta_part,
-                                                      uint32_t rep)
+constexpr int kMetroLoci = 256;   // loci of P staged in LDS at a time
+
+// Biallelic 2-bit path, K templated: the block's P columns are staged in LDS
+// (wave-uniform reads), four loci per genotype load, fused multiply-adds in
+// the dot products (the ratio feeds an exp() compared with a uniform; it
+// needs no reference-exact rounding).
+template <int K>
+__global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(ModelArgs m,
+                                                           const uint32_t *__restrict__ geno,
+                                                           int Lpad, const double *__restrict__ Q,
+                                                           const double *__restrict__ P,
+                                                           const double *__restrict__ alpha,
+                                                           int loci_per_range,
+                                                           double *__restrict__ delta_part,
+                                                           const uint32_t *__restrict__ repp)
 {
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    __shared__ double s_p[kMetroLoci * K * 2];
+    const int i = blockIdx.x * kNT + threadIdx.x;
+    const bool active = i < m.N;
+    double tq[K], cq[K];
+    {
+        double tmp[kMaxK];
+        if (active) {
+            dirichlet_q(m, alpha, nullptr, rep, i, PH_QMETRO, tmp);
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            tq[k] = active ? tmp[k] : 1.0;
+            cq[k] = active ? Q[(size_t)i * K + k] : 1.0;
+        }
+    }
+    const int l0 = blockIdx.y * loci_per_range;
+    const int l1 = min(l0 + loci_per_range, m.L);
+    const uint32_t *row = geno + (size_t)(active ? (i >> 4) : 0) * Lpad;
+    const int sh = 2 * (i & 15);
+    double pp = 1.0, pm = 1.0;
+    int ep = 0, em = 0;
+    for (int c0 = l0; c0 < l1; c0 += kMetroLoci) {
+        const int c1 = min(c0 + kMetroLoci, l1);
+        __syncthreads();
+        for (int t = threadIdx.x; t < (c1 - c0) * K * 2; t += kNT) {
+            s_p[t] = P[(size_t)c0 * K * 2 + t];
+        }
+        __syncthreads();
+        for (int lb = c0; lb < c1; lb += 4) {
+            const uint4 w4 = *(const uint4 *)(row + lb);
+            const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int l = lb + u;
+                if (l < c1) {
+                    const double *Pl = s_p + (l - c0) * K * 2;
+                    double tp0 = 0.0, tp1 = 0.0, tm0 = 0.0, tm1 = 0.0;
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const double p0 = Pl[2 * k], p1 = Pl[2 * k + 1];
+                        tp0 = fma(tq[k], p0, tp0);
+                        tp1 = fma(tq[k], p1, tp1);
+                        tm0 = fma(cq[k], p0, tm0);
+                        tm1 = fma(cq[k], p1, tm1);
+                    }
+                    const uint32_t code = (ws[u] >> sh) & 3u;
+                    if (code != 3u) {
+                        const bool x0 = code == 2u, x1 = code != 0u;
+                        pp *= x0 ? tp1 : tp0;
+                        pp *= x1 ? tp1 : tp0;
+                        pm *= x0 ? tm1 : tm0;
+                        pm *= x1 ? tm1 : tm0;
+                    }
+                }
+            }
+            int e;
+            pp = frexp(pp, &e);
+            ep += e;
+            pm = frexp(pm, &e);
+            em += e;
+        }
+    }
+    if (active) {
+        delta_part[(size_t)blockIdx.y * m.N + i] = (log(pp) - log(pm)) + (double)(ep - em) * kLn2;
+    }
+}
+
+// Byte-per-copy path (any allele count, runtime K).
+__global__ __launch_bounds__(kNT) void k_metro_ratio_byte(ModelArgs m,
+                                                           const uint32_t *__restrict__ geno,
+                                                           int Lpad, const double *__restrict__ Q,
+                                                           const double *__restrict__ P,
+                                                           const double *__restrict__ alpha,
+                                                           int loci_per_range,
+                                                           double *__restrict__ delta_part,
+                                                           const uint32_t *__restrict__ repp)
+{
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const int i = blockIdx.x * kNT + threadIdx.x;
     if (i >= m.N) {
         return;
@@ -220,26 +338,15 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio(ModelArgs m, const uint32_t
     double pp = 1.0, pm = 1.0;
     int ep = 0, em = 0;
     for (int l = l0; l < l1; l++) {
-        int xa, xb;
-        if (FMT == 1) {
-            const uint32_t word = geno[(size_t)(i >> 4) * Lpad + l];
-            const uint32_t code = (word >> (2 * (i & 15))) & 3u;
-            if (code == 3u) {
-                continue;
-            }
-            xa = code == 2u;
-            xb = code >= 1u;
-        } else {
-            const uint32_t word = geno[(size_t)(i >> 1) * Lpad + l];
-            xa = (int)((word >> (16 * (i & 1))) & 0xffu);
-            xb = (int)((word >> (16 * (i & 1) + 8)) & 0xffu);
-        }
+        const uint32_t word = geno[(size_t)(i >> 1) * Lpad + l];
+        const int xa = (int)((word >> (16 * (i & 1))) & 0xffu);
+        const int xb = (int)((word >> (16 * (i & 1) + 8)) & 0xffu);
         const double *Pl = P + (size_t)l * K * m.A;
         if (xa != 0xff) {
             double tp = 0.0, tm = 0.0;
             for (int k = 0; k < K; k++) {
-                tp += tq[k] * Pl[k * m.A + xa];
-                tm += cq[k] * Pl[k * m.A + xa];
+                tp = fma(tq[k], Pl[k * m.A + xa], tp);
+                tm = fma(cq[k], Pl[k * m.A + xa], tm);
             }
             pp *= tp;
             pm *= tm;
@@ -247,8 +354,8 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio(ModelArgs m, const uint32_t
         if (xb != 0xff) {
             double tp = 0.0, tm = 0.0;
             for (int k = 0; k < K; k++) {
-                tp += tq[k] * Pl[k * m.A + xb];
-                tm += cq[k] * Pl[k * m.A + xb];
+                tp = fma(tq[k], Pl[k * m.A + xb], tp);
+                tm = fma(cq[k], Pl[k * m.A + xb], tm);
             }
             pp *= tp;
             pm *= tm;
@@ -268,8 +375,9 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(ModelArgs m, int n_ranges,
                                                        double *__restrict__ Q,
                                                        double *__restrict__ QSum, int collect,
                                                        double *__restrict__ logq_part,
-                                                       int32_t *__restrict__ accepted, uint32_t rep)
+                                                       int32_t *__restrict__ accepted, const uint32_t *__restrict__ repp)
 {
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     __shared__ double s_red[kNT / 64];
     const int i = blockIdx.x * kNT + threadIdx.x;
     const bool active = i < m.N;
@@ -293,7 +401,7 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(ModelArgs m, int n_ranges,
             accepted[i] = acc;
         }
     }
-    finish_q(m, i, active, q, Q, QSum, collect, logq_part, s_red);
+    finish_q<kNT>(m, i, active, q, Q, QSum, collect, logq_part, s_red);
 }
 
 // ---------------------------------------------------------------------------
@@ -304,8 +412,9 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(ModelArgs m, int n_ranges,
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void k_update_alpha(ModelArgs m, const double *__restrict__ logq_part,
                                                        int nparts, double *__restrict__ alpha,
-                                                       uint32_t rep)
+                                                       const uint32_t *__restrict__ repp)
 {
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     __shared__ double s_red[kNT / 64];
     __shared__ double s_stat[kMaxK + 1];
     for (int k = 0; k <= m.K; k++) {
@@ -409,8 +518,9 @@ __global__ __launch_bounds__(kNT) void k_lambda_partial(ModelArgs m, const int32
 __global__ __launch_bounds__(kNT) void k_lambda_finalize(ModelArgs m, const double *__restrict__ part,
                                                           int nparts, const int32_t *__restrict__ ahist,
                                                           int nhist, double *__restrict__ lambda,
-                                                          uint32_t rep)
+                                                          const uint32_t *__restrict__ repp)
 {
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     __shared__ double s_red[kNT / 64];
     __shared__ double s_stat[kMaxK];
     const int nsums = m.popspecificlambda ? m.K : 1;
@@ -469,8 +579,9 @@ __global__ __launch_bounds__(kNT) void k_update_eps(ModelArgs m, const int32_t *
                                                      const double *__restrict__ lambda,
                                                      double *__restrict__ eps,
                                                      double *__restrict__ epsSum, int collect,
-                                                     uint32_t rep)
+                                                     const uint32_t *__restrict__ repp)
 {
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const int l = blockIdx.x * kNT + threadIdx.x;
     if (l >= m.L) {
         return;
@@ -536,8 +647,9 @@ __global__ __launch_bounds__(kNT) void k_fst_partial(ModelArgs m, const int32_t 
                                                       const double *__restrict__ eps,
                                                       const double *__restrict__ P,
                                                       const double *__restrict__ F,
-                                                      double *__restrict__ part, uint32_t rep)
+                                                      double *__restrict__ part, const uint32_t *__restrict__ repp)
 {
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     __shared__ double s_red[kNT / 64];
     const int l = blockIdx.x * kNT + threadIdx.x;
     const int na = (l < m.L) ? nall[l] : 0;
@@ -568,8 +680,9 @@ __global__ __launch_bounds__(kNT) void k_fst_partial(ModelArgs m, const int32_t 
 
 __global__ __launch_bounds__(kNT) void k_fst_finalize(ModelArgs m, const double *__restrict__ part,
                                                        int nparts, int nloci_nonzero,
-                                                       double *__restrict__ F, uint32_t rep)
+                                                       double *__restrict__ F, const uint32_t *__restrict__ repp)
 {
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     __shared__ double s_red[kNT / 64];
     __shared__ double s_stat[kMaxK];
     const int numpops1 = m.onefst ? 1 : m.K;

@@ -38,6 +38,25 @@ __device__ __forceinline__ void add_ind_counts(uint32_t *s_cnt, int slot, int K,
     }
 }
 
+#ifndef SGPU_ZVAR
+#define SGPU_ZVAR 1
+#endif
+
+// v + (bit of this lane in mask): one carry-in add, the mask staying in SGPRs.
+__device__ __forceinline__ uint32_t add_lane_bit(uint32_t v, uint64_t mask)
+{
+    uint32_t r;
+    uint64_t carry_out;
+    asm("v_addc_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(r), "=s"(carry_out) : "v"(v), "s"(mask));
+    return r;
+}
+
+// Parks a wave-uniform value in lane lane_idx of a collector register.
+__device__ __forceinline__ uint32_t write_lane(uint32_t v, uint32_t value, int lane_idx)
+{
+    return ((int)(threadIdx.x & 63) == lane_idx) ? value : v;
+}
+
 // Running product of clamped terms with exponent bookkeeping (CalcLikeInd's
 // running product, CalcLike.c:113-126, without the periodic logs).
 struct LikeAcc {
@@ -55,18 +74,40 @@ struct LikeAcc {
 
 // ---------------------------------------------------------------------------
 // Biallelic 2-bit path.  K is a template parameter (registers, unrolled dots).
+//
+// Per lane (locus) and individual, with the pop-k prefix sums pre_a[k] of
+// allele a and r_c = u_c * S_c for copy c:  z_c > k  <=>  pre_{x_c}[k] < r_c.
+// The kernel only ever forms these "z > k" predicates:
+//   per-locus counters   T[k]  = #copies with z > k, A1[k] = #allele-1 copies
+//                        with z > k (lane-private registers, carry-in adds);
+//   per-individual       popcount of the wave ballot of each predicate (SALU),
+//                        parked in lane t of a collector register and added
+//                        into LDS once per 16 individuals.
+// Counts per population are differences of consecutive "z > k" counts.
+// Q and P reach the kernel through restrict pointers, so Q[i,:] (wave
+// uniform) is fetched with scalar loads.
 // ---------------------------------------------------------------------------
-template <int K, bool INIT, bool LIKE>
-__global__ __launch_bounds__(kZLoci) void k_update_z_snp2(ZArgs a)
+struct ZDims {
+    int N, L, A, Lpad, G, rows_per_tile;
+    uint32_t seed, chain;
+};
+
+template <int K, bool INIT, bool LIKE, bool ZOUT>
+__global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
+    const uint32_t *__restrict__ geno, const double *__restrict__ Q,
+    const double *__restrict__ P, int32_t *__restrict__ nApart, int32_t *__restrict__ nQpart,
+    double *__restrict__ likepart, int8_t *__restrict__ zout, ZDims d,
+    const uint32_t *__restrict__ repp)
 {
-    extern __shared__ uint32_t s_cnt[];   // [tile individuals][K]
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    extern __shared__ uint32_t s_cnt[];   // [tile individual][K]: valid copies, then z > k
     const int lane = threadIdx.x & 63;
     const int l = blockIdx.x * kZLoci + threadIdx.x;   // < Lpad by construction
-    const bool lok = l < a.L;
-    const int row0 = blockIdx.y * a.rows_per_tile;
-    const int row1 = min(row0 + a.rows_per_tile, a.G);
+    const bool lok = l < d.L;
+    const int row0 = blockIdx.y * d.rows_per_tile;
+    const int row1 = min(row0 + d.rows_per_tile, d.G);
     const int ibase = row0 * 16;
-    const int ntile = min((row1 - row0) * 16, a.N - ibase);
+    const int ntile = min((row1 - row0) * 16, d.N - ibase);
 
     for (int t = threadIdx.x; t < ntile * K; t += kZLoci) {
         s_cnt[t] = 0;
@@ -79,49 +120,73 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(ZArgs a)
         P0[k] = 0.0;
         P1[k] = 0.0;
         if (!INIT && lok) {
-            P0[k] = a.P[((size_t)l * K + k) * a.A + 0];
-            P1[k] = a.P[((size_t)l * K + k) * a.A + 1];
+            P0[k] = P[((size_t)l * K + k) * d.A + 0];
+            P1[k] = P[((size_t)l * K + k) * d.A + 1];
         }
     }
-    int cnt0[K], cnt1[K];
+    uint32_t T[K], A1[K];   // entries 0..K-2 used
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        cnt0[k] = 0;
-        cnt1[k] = 0;
+        T[k] = 0;
+        A1[k] = 0;
     }
+    uint32_t Ttot = 0, A1tot = 0;
     LikeAcc acc;
 
+    uint32_t word = geno[(size_t)row0 * d.Lpad + l];
     for (int g = row0; g < row1; g++) {
-        const uint32_t word = a.geno[(size_t)g * a.Lpad + l];
+        const uint32_t next = (g + 1 < row1) ? geno[(size_t)(g + 1) * d.Lpad + l] : 0u;
+        uint32_t col[K];   // collector: lane t holds the counts of individual 16g+t
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            col[k] = 0;
+        }
         for (int jp = 0; jp < 8; jp++) {
             const int i0 = g * 16 + 2 * jp;
-            if (i0 >= a.N) {
+            if (i0 >= d.N) {
                 break;
             }
-            const u32x4 w = philox4x32_10(a.rep, (uint32_t)l, (uint32_t)(i0 >> 1),
-                                          (INIT ? PH_INITZ : PH_Z) << 24, a.seed, a.chain);
+            const u32x4 w = philox4x32_10(rep, (uint32_t)l, (uint32_t)(i0 >> 1),
+                                          (INIT ? PH_INITZ : PH_Z) << 24, d.seed, d.chain);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int i = i0 + h;
-                if (i >= a.N) {
+                if (i >= d.N) {
                     break;
                 }
-                const uint32_t code = (word >> (2 * (2 * jp + h))) & 3u;
+                const int t = 2 * jp + h;
+                const uint32_t code = (word >> (2 * t)) & 3u;
                 const bool valid = code != 3u;
-                const int x0 = code == 2u;
-                const int x1 = code >= 1u;
+                const bool x0 = code == 2u;
+                const bool x1 = code != 0u;
                 const uint32_t wa = h ? w.z : w.x;
                 const uint32_t wb = h ? w.w : w.y;
+                Ttot += valid;
+                Ttot += valid;
+                A1tot += valid && x0;
+                A1tot += valid && x1;
+                col[0] = write_lane(col[0], 2u * (uint32_t)__popcll(__ballot(valid)), t);
                 int z0 = 0, z1 = 0;
                 if (INIT) {
-                    z0 = (int)(((uint64_t)wa * (uint32_t)K) >> 32);
-                    z1 = (int)(((uint64_t)wb * (uint32_t)K) >> 32);
+                    const uint32_t u0 = (uint32_t)(((uint64_t)wa * (uint32_t)K) >> 32);
+                    const uint32_t u1 = (uint32_t)(((uint64_t)wb * (uint32_t)K) >> 32);
+#pragma unroll
+                    for (int k = 0; k < K - 1; k++) {
+                        const bool m0 = valid && u0 > (uint32_t)k;
+                        const bool m1 = valid && u1 > (uint32_t)k;
+                        T[k] += m0;
+                        T[k] += m1;
+                        A1[k] += m0 && x0;
+                        A1[k] += m1 && x1;
+                        const uint32_t n = (uint32_t)(__popcll(__ballot(m0)) + __popcll(__ballot(m1)));
+                        col[k + 1] = write_lane(col[k + 1], n, t);
+                    }
                 } else {
                     double pre0[K], pre1[K];
                     double s0 = 0.0, s1 = 0.0;
 #pragma unroll
                     for (int k = 0; k < K; k++) {
-                        const double q = a.Q[(size_t)i * K + k];
+                        const double q = Q[(size_t)i * K + k];
                         const double c0 = q * P0[k];
                         const double c1 = q * P1[k];
                         s0 = s0 + c0;
@@ -129,57 +194,105 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(ZArgs a)
                         pre0[k] = s0;
                         pre1[k] = s1;
                     }
-                    const double S0 = x0 ? s1 : s0;
-                    const double S1 = x1 ? s1 : s0;
-                    const double r0 = uz(wa) * S0;
-                    const double r1 = uz(wb) * S1;
+                    if (LIKE && valid) {
+                        acc.mul(x0 ? s1 : s0);
+                        acc.mul(x1 ? s1 : s0);
+                    }
+                    // r = (w * 2^-32) * S, formed as w * (S * 2^-32): the power-of-two
+                    // scaling is exact, so both orders round the same product once.
+                    // Missing copies get r = -1 and never count.
+                    const double h0 = s0 * 0x1p-32, h1 = s1 * 0x1p-32;
+                    const double r0 = valid ? (double)wa * (x0 ? h1 : h0) : -1.0;
+                    const double r1 = valid ? (double)wb * (x1 ? h1 : h0) : -1.0;
+#if SGPU_ZVAR
+                    // both allele sequences are compared and the copy's allele picks
+                    // the lane mask in scalar registers
+                    const uint64_t xm0 = __ballot(x0), xm1 = __ballot(x1);
 #pragma unroll
                     for (int k = 0; k < K - 1; k++) {
-                        z0 += ((x0 ? pre1[k] : pre0[k]) < r0);
-                        z1 += ((x1 ? pre1[k] : pre0[k]) < r1);
+                        const uint64_t a0 = __ballot(pre0[k] < r0), b0 = __ballot(pre1[k] < r0);
+                        const uint64_t a1 = __ballot(pre0[k] < r1), b1 = __ballot(pre1[k] < r1);
+                        const uint64_t m0 = (xm0 & b0) | (~xm0 & a0);   // z0 > k
+                        const uint64_t m1 = (xm1 & b1) | (~xm1 & a1);   // z1 > k
+                        T[k] = add_lane_bit(add_lane_bit(T[k], m0), m1);
+                        A1[k] = add_lane_bit(add_lane_bit(A1[k], m0 & xm0), m1 & xm1);
+                        const uint32_t n = (uint32_t)(__popcll(m0) + __popcll(m1));
+                        col[k + 1] = write_lane(col[k + 1], n, t);
+                        if (ZOUT) {
+                            z0 += (int)((m0 >> lane) & 1u);
+                            z1 += (int)((m1 >> lane) & 1u);
+                        }
                     }
-                    if (LIKE && valid) {
-                        acc.mul(S0);
-                        acc.mul(S1);
-                    }
-                }
-                if (valid) {
+#else
 #pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        cnt0[k] += (z0 == k && !x0) + (z1 == k && !x1);
-                        cnt1[k] += (z0 == k && x0) + (z1 == k && x1);
+                    for (int k = 0; k < K - 1; k++) {
+                        const bool m0 = x0 ? (pre1[k] < r0) : (pre0[k] < r0);   // z0 > k
+                        const bool m1 = x1 ? (pre1[k] < r1) : (pre0[k] < r1);   // z1 > k
+                        T[k] += m0;
+                        T[k] += m1;
+                        A1[k] += m0 && x0;
+                        A1[k] += m1 && x1;
+                        const uint32_t n = (uint32_t)(__popcll(__ballot(m0)) + __popcll(__ballot(m1)));
+                        col[k + 1] = write_lane(col[k + 1], n, t);
+                        if (ZOUT) {
+                            z0 += m0;
+                            z1 += m1;
+                        }
                     }
+#endif
                 }
-                add_ind_counts(s_cnt, i - ibase, K, lane, z0, z1, valid, valid);
-                if (a.zout && lok) {
-                    a.zout[((size_t)i * 2 + 0) * a.L + l] = valid ? (int8_t)z0 : (int8_t)-9;
-                    a.zout[((size_t)i * 2 + 1) * a.L + l] = valid ? (int8_t)z1 : (int8_t)-9;
+                if (ZOUT && lok) {
+                    zout[((size_t)i * 2 + 0) * d.L + l] = valid ? (int8_t)z0 : (int8_t)-9;
+                    zout[((size_t)i * 2 + 1) * d.L + l] = valid ? (int8_t)z1 : (int8_t)-9;
                 }
             }
             if (LIKE) {
                 acc.renorm();
             }
         }
+        // add this row's per-individual counts into LDS (lanes 0..15 of each wave)
+        const int t0 = g * 16 - ibase;
+        if (lane < 16 && t0 + lane < ntile) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                atomicAdd(&s_cnt[(t0 + lane) * K + k], col[k]);
+            }
+        }
+        word = next;
     }
 
     if (lok) {
-        int32_t *dst = a.nApart + ((size_t)blockIdx.y * a.L + l) * K * a.A;
+        int32_t *dst = nApart + ((size_t)blockIdx.y * d.L + l) * K * d.A;
+        uint32_t prevT = Ttot, prevA = A1tot;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            dst[k * a.A + 0] = cnt0[k];
-            dst[k * a.A + 1] = cnt1[k];
+            const uint32_t tk = (k < K - 1) ? T[k] : 0u;
+            const uint32_t ak = (k < K - 1) ? A1[k] : 0u;
+            const uint32_t n1 = prevA - ak;
+            const uint32_t nt = prevT - tk;
+            dst[k * d.A + 0] = (int32_t)(nt - n1);
+            dst[k * d.A + 1] = (int32_t)n1;
+            prevT = tk;
+            prevA = ak;
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < ntile * K; t += kZLoci) {
-        a.nQpart[((size_t)blockIdx.x * a.N + ibase) * K + t] = (int32_t)s_cnt[t];
+    // convert "valid, z > 0, ..., z > K-2" into counts per population
+    for (int t = threadIdx.x; t < ntile; t += kZLoci) {
+        int32_t *dst = nQpart + ((size_t)blockIdx.x * d.N + ibase + t) * K;
+        uint32_t prev = s_cnt[t * K + 0];
+        for (int k = 0; k < K; k++) {
+            const uint32_t gk = (k < K - 1) ? s_cnt[t * K + 1 + k] : 0u;
+            dst[k] = (int32_t)(prev - gk);
+            prev = gk;
+        }
     }
     if (LIKE) {
         __shared__ double s_red[kZLoci / 64];
         const double v = lok ? acc.log_value() : 0.0;
         const double tot = block_sum<kZLoci>(v, s_red);
         if (threadIdx.x == 0) {
-            a.likepart[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
+            likepart[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
         }
     }
 }
@@ -190,8 +303,9 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(ZArgs a)
 // lane owns its locus's partial histogram row and updates it in place.
 // ---------------------------------------------------------------------------
 template <bool INIT, bool LIKE>
-__global__ __launch_bounds__(kZLoci) void k_update_z_byte(ZArgs a)
+__global__ __launch_bounds__(kZLoci) void k_update_z_byte(ZArgs a, const uint32_t *__restrict__ repp)
 {
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     extern __shared__ uint32_t s_cnt[];
     const int lane = threadIdx.x & 63;
     const int K = a.K;
@@ -218,7 +332,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_byte(ZArgs a)
     for (int g = row0; g < row1; g++) {
         const uint32_t word = a.geno[(size_t)g * a.Lpad + l];
         const int i0 = g * 2;
-        const u32x4 w = philox4x32_10(a.rep, (uint32_t)l, (uint32_t)(i0 >> 1),
+        const u32x4 w = philox4x32_10(rep, (uint32_t)l, (uint32_t)(i0 >> 1),
                                       (INIT ? PH_INITZ : PH_Z) << 24, a.seed, a.chain);
         for (int h = 0; h < 2; h++) {
             const int i = i0 + h;

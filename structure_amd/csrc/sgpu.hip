@@ -65,17 +65,24 @@ struct sgpu_ctx {
     int n_ranges = 0, loci_per_range = 0;
     // partial-buffer block counts
     int q_blocks = 0, l_blocks = 0;
+    int logq_rows = 0;   // rows of logq_part written by the last Q update
     // device state
     uint32_t *geno = nullptr;
     int32_t *nall = nullptr, *ahist = nullptr;
     double *Q = nullptr, *P = nullptr, *eps = nullptr, *F = nullptr, *alpha = nullptr,
            *lambda = nullptr;
-    int32_t *nApart = nullptr, *nQpart = nullptr, *nA = nullptr, *nQ = nullptr;
+    int32_t *nApart = nullptr, *nQpart = nullptr, *nA = nullptr, *nQ = nullptr, *nQfold = nullptr;
     double *likepart = nullptr, *logq_part = nullptr, *delta_part = nullptr,
            *lam_part = nullptr, *fst_part = nullptr;
     double *QSum = nullptr, *PSum = nullptr, *epsSum = nullptr, *FstSum = nullptr,
            *alphaSum = nullptr, *lambdaSum = nullptr, *like3 = nullptr;
     int8_t *zdbg = nullptr;
+    uint32_t *repd = nullptr;     // device copy of the sweep index read by every kernel
+    uint32_t rep_shadow = 0;      // host view of *repd
+    bool rep_known = false;
+    bool capturing = false;
+    bool use_graphs = true;
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};   // [metro | 2*collect]
     int nhist = 0, nloci_nonzero = 0;
     int64_t nsum = 0;
     bool initialized = false;
@@ -152,22 +159,22 @@ ZArgs z_args(sgpu_ctx *c, uint32_t rep)
     a.inds_per_row = c->inds_per_row;
     a.seed = c->m.seed;
     a.chain = c->m.chain;
-    a.rep = rep;
     return a;
 }
 
-typedef void (*zkern_t)(ZArgs);
+typedef void (*zkern_t)(const uint32_t *, const double *, const double *, int32_t *, int32_t *,
+                        double *, int8_t *, ZDims, const uint32_t *);
 
 template <int K>
 zkern_t pick_snp2(int mode)
 {
-    if (mode == 0) {
-        return k_update_z_snp2<K, true, false>;
+    switch (mode) {
+    case 0: return k_update_z_snp2<K, true, false, false>;
+    case 1: return k_update_z_snp2<K, false, false, false>;
+    case 2: return k_update_z_snp2<K, false, true, false>;
+    case 3: return k_update_z_snp2<K, false, false, true>;
+    default: return k_update_z_snp2<K, false, true, true>;
     }
-    if (mode == 1) {
-        return k_update_z_snp2<K, false, false>;
-    }
-    return k_update_z_snp2<K, false, true>;
 }
 
 zkern_t snp2_kernel(int K, int mode)
@@ -193,25 +200,64 @@ zkern_t snp2_kernel(int K, int mode)
     }
 }
 
-// mode 0 = initial Z, 1 = update, 2 = update + lnP terms
+typedef void (*mkern_t)(ModelArgs, const uint32_t *, int, const double *, const double *,
+                        const double *, int, double *, const uint32_t *);
+
+mkern_t metro_snp2_kernel(int K)
+{
+    switch (K) {
+    case 1: return k_metro_ratio_snp2<1>;
+    case 2: return k_metro_ratio_snp2<2>;
+    case 3: return k_metro_ratio_snp2<3>;
+    case 4: return k_metro_ratio_snp2<4>;
+    case 5: return k_metro_ratio_snp2<5>;
+    case 6: return k_metro_ratio_snp2<6>;
+    case 7: return k_metro_ratio_snp2<7>;
+    case 8: return k_metro_ratio_snp2<8>;
+    case 9: return k_metro_ratio_snp2<9>;
+    case 10: return k_metro_ratio_snp2<10>;
+    case 11: return k_metro_ratio_snp2<11>;
+    case 12: return k_metro_ratio_snp2<12>;
+    case 13: return k_metro_ratio_snp2<13>;
+    case 14: return k_metro_ratio_snp2<14>;
+    case 15: return k_metro_ratio_snp2<15>;
+    case 16: return k_metro_ratio_snp2<16>;
+    default: return nullptr;
+    }
+}
+
+// mode 0 = initial Z, 1 = update, 2 = update + lnP terms (+2 with a debug copy of Z)
 int launch_z(sgpu_ctx *c, uint32_t rep, int mode, int8_t *zdev)
 {
     ZArgs a = z_args(c, rep);
     a.zout = zdev;
     dim3 grid(c->n_lt, c->n_it), block(kZLoci);
     if (c->fmt == SGPU_GENO_SNP2) {
-        zkern_t k = snp2_kernel(c->m.K, mode);
+        zkern_t k = snp2_kernel(c->m.K, (zdev && mode > 0) ? mode + 2 : mode);
         if (!k) {
             return fail(SGPU_ERR_UNSUPPORTED, "no SNP2 UpdateZ kernel for K=%d", c->m.K);
         }
-        hipLaunchKernelGGL(k, grid, block, c->z_lds, c->stream, a);
+        ZDims zd{};
+        zd.N = c->m.N;
+        zd.L = c->m.L;
+        zd.A = c->m.A;
+        zd.Lpad = c->Lpad;
+        zd.G = c->G;
+        zd.rows_per_tile = c->rows_per_tile;
+        zd.seed = c->m.seed;
+        zd.chain = c->m.chain;
+            hipLaunchKernelGGL(k, grid, block, c->z_lds, c->stream, c->geno, c->Q, c->P, c->nApart,
+                           c->nQpart, c->likepart, zdev, zd, (const uint32_t *)c->repd);
     } else {
         if (mode == 0) {
-            hipLaunchKernelGGL((k_update_z_byte<true, false>), grid, block, c->z_lds, c->stream, a);
+            hipLaunchKernelGGL((k_update_z_byte<true, false>), grid, block, c->z_lds, c->stream, a,
+                               (const uint32_t *)c->repd);
         } else if (mode == 1) {
-            hipLaunchKernelGGL((k_update_z_byte<false, false>), grid, block, c->z_lds, c->stream, a);
+            hipLaunchKernelGGL((k_update_z_byte<false, false>), grid, block, c->z_lds, c->stream, a,
+                               (const uint32_t *)c->repd);
         } else {
-            hipLaunchKernelGGL((k_update_z_byte<false, true>), grid, block, c->z_lds, c->stream, a);
+            hipLaunchKernelGGL((k_update_z_byte<false, true>), grid, block, c->z_lds, c->stream, a,
+                               (const uint32_t *)c->repd);
         }
     }
     LAUNCH_OK();
@@ -261,6 +307,7 @@ int launch_logq(sgpu_ctx *c)
     hipLaunchKernelGGL(k_logq_stats, dim3(c->q_blocks), dim3(kNT), 0, c->stream, c->ma, c->Q,
                        c->logq_part);
     LAUNCH_OK();
+    c->logq_rows = c->q_blocks;
     return SGPU_OK;
 }
 
@@ -284,6 +331,11 @@ int drain_timing(sgpu_ctx *c)
         return SGPU_OK;
     }
     HIP_OK(hipStreamSynchronize(c->stream));
+    for (int t = 0; t < 4; t++) {
+        if (c->graph[t]) {
+            (void)hipGraphExecDestroy(c->graph[t]);
+        }
+    }
     for (auto &p : c->pending) {
         float ms = 0.f;
         HIP_OK(hipEventElapsedTime(&ms, p.a, p.b));
@@ -324,6 +376,23 @@ struct PhaseTimer {
         }
     }
 };
+
+__global__ void k_set_rep(uint32_t *repd, uint32_t v) { repd[0] = v; }
+__global__ void k_advance_rep(uint32_t *repd) { repd[0] += 1u; }
+
+// Make the device sweep index equal rep (no-op while capturing a graph, whose
+// kernels read the index that the previous replay advanced).
+int use_rep(sgpu_ctx *c, uint32_t rep)
+{
+    if (c->capturing || (c->rep_known && c->rep_shadow == rep)) {
+        return SGPU_OK;
+    }
+    hipLaunchKernelGGL(k_set_rep, dim3(1), dim3(1), 0, c->stream, c->repd, rep);
+    LAUNCH_OK();
+    c->rep_shadow = rep;
+    c->rep_known = true;
+    return SGPU_OK;
+}
 
 int check_ready(sgpu_ctx *c)
 {
@@ -452,7 +521,7 @@ int sgpu_create(const sgpu_model *m, const sgpu_data *d, sgpu_ctx **out)
     c->q_blocks = (int)cdiv(N, kNT);
     c->l_blocks = (int)cdiv(L, kNT);
     c->n_ranges = (int)std::max<size_t>(1, std::min<size_t>(cdiv(2048, c->q_blocks), L));
-    c->loci_per_range = (int)cdiv(L, c->n_ranges);
+    c->loci_per_range = (int)(cdiv(cdiv(L, c->n_ranges), 4) * 4);   // multiple of 4 loci
     c->n_ranges = (int)cdiv(L, c->loci_per_range);
 
     if ((rc = dalloc(c, &c->geno, need)) || (rc = dalloc(c, &c->nall, L)) ||
@@ -463,15 +532,16 @@ int sgpu_create(const sgpu_model *m, const sgpu_data *d, sgpu_ctx **out)
         (rc = dalloc(c, &c->nApart, (size_t)c->n_it * L * K * A)) ||
         (rc = dalloc(c, &c->nQpart, (size_t)c->n_lt * N * K)) ||
         (rc = dalloc(c, &c->nA, (size_t)L * K * A)) || (rc = dalloc(c, &c->nQ, (size_t)N * K)) ||
+        (rc = dalloc(c, &c->nQfold, (size_t)kRowSplit * N * K)) ||
         (rc = dalloc(c, &c->likepart, (size_t)c->n_it * c->n_lt)) ||
-        (rc = dalloc(c, &c->logq_part, (size_t)c->q_blocks * (K + 1))) ||
+        (rc = dalloc(c, &c->logq_part, cdiv(N, kGibbsInds) * (K + 1))) ||
         (rc = dalloc(c, &c->delta_part, (size_t)c->n_ranges * N)) ||
         (rc = dalloc(c, &c->lam_part, (size_t)c->l_blocks * K)) ||
         (rc = dalloc(c, &c->fst_part, (size_t)c->l_blocks * K)) ||
         (rc = dalloc(c, &c->QSum, (size_t)N * K)) || (rc = dalloc(c, &c->PSum, (size_t)L * K * A)) ||
         (rc = dalloc(c, &c->epsSum, (size_t)L * A)) || (rc = dalloc(c, &c->FstSum, K)) ||
         (rc = dalloc(c, &c->alphaSum, K)) || (rc = dalloc(c, &c->lambdaSum, K)) ||
-        (rc = dalloc(c, &c->like3, 3))) {
+        (rc = dalloc(c, &c->like3, 3)) || (rc = dalloc(c, &c->repd, 2))) {
         return cleanup(rc);
     }
     // allele-count histogram over loci with more than one allele (lambda update)
@@ -527,6 +597,11 @@ int sgpu_destroy(sgpu_ctx *c)
     if (c->zdbg) {
         (void)hipFree(c->zdbg);
     }
+    for (int t = 0; t < 4; t++) {
+        if (c->graph[t]) {
+            (void)hipGraphExecDestroy(c->graph[t]);
+        }
+    }
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
@@ -573,8 +648,9 @@ int sgpu_init_state(sgpu_ctx *c)
     hipLaunchKernelGGL(k_init_eps, dim3(cdiv(L, 256)), dim3(256), 0, c->stream, c->ma, c->fmt,
                        c->geno, c->Lpad, c->nall, m.lambda0, c->eps);
     LAUNCH_OK();
-    int rc = launch_z(c, 0, 0, nullptr);
-    if (rc) {
+    c->rep_known = false;
+    int rc = use_rep(c, 0);
+    if (rc || (rc = launch_z(c, 0, 0, nullptr))) {
         return rc;
     }
     if ((rc = launch_logq(c))) {
@@ -655,7 +731,7 @@ int sgpu_set_state(sgpu_ctx *c, int32_t which, const void *host, size_t bytes)
 int sgpu_update_p(sgpu_ctx *c, uint32_t rep, int32_t collect)
 {
     int rc = check_ready(c);
-    if (rc) {
+    if (rc || (rc = use_rep(c, rep))) {
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_P);
@@ -663,11 +739,11 @@ int sgpu_update_p(sgpu_ctx *c, uint32_t rep, int32_t collect)
     if (c->m.A <= 8) {
         hipLaunchKernelGGL(k_update_p<8>, dim3(cdiv(nthreads, kNT)), dim3(kNT), 0, c->stream, c->ma,
                            c->nall, c->nApart, c->n_it, c->eps, c->F, c->lambda, c->P, c->PSum,
-                           (int)collect, rep);
+                           (int)collect, (const uint32_t *)c->repd);
     } else {
         hipLaunchKernelGGL(k_update_p<0>, dim3(cdiv(nthreads, kNT)), dim3(kNT), 0, c->stream, c->ma,
                            c->nall, c->nApart, c->n_it, c->eps, c->F, c->lambda, c->P, c->PSum,
-                           (int)collect, rep);
+                           (int)collect, (const uint32_t *)c->repd);
     }
     LAUNCH_OK();
     return SGPU_OK;
@@ -676,7 +752,7 @@ int sgpu_update_p(sgpu_ctx *c, uint32_t rep, int32_t collect)
 int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
 {
     int rc = check_ready(c);
-    if (rc) {
+    if (rc || (rc = use_rep(c, rep))) {
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_Q);
@@ -684,20 +760,30 @@ int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
     if (metro) {
         dim3 grid(c->q_blocks, c->n_ranges);
         if (c->fmt == SGPU_GENO_SNP2) {
-            hipLaunchKernelGGL(k_metro_ratio<1>, grid, dim3(kNT), 0, c->stream, c->ma, c->geno,
-                               c->Lpad, c->Q, c->P, c->alpha, c->loci_per_range, c->delta_part, rep);
+            mkern_t k = metro_snp2_kernel(c->m.K);
+            if (!k) {
+                return fail(SGPU_ERR_UNSUPPORTED, "no SNP2 Metropolis kernel for K=%d", c->m.K);
+            }
+            hipLaunchKernelGGL(k, grid, dim3(kNT), 0, c->stream, c->ma, c->geno, c->Lpad, c->Q,
+                               c->P, c->alpha, c->loci_per_range, c->delta_part, (const uint32_t *)c->repd);
         } else {
-            hipLaunchKernelGGL(k_metro_ratio<2>, grid, dim3(kNT), 0, c->stream, c->ma, c->geno,
-                               c->Lpad, c->Q, c->P, c->alpha, c->loci_per_range, c->delta_part, rep);
+            hipLaunchKernelGGL(k_metro_ratio_byte, grid, dim3(kNT), 0, c->stream, c->ma, c->geno,
+                               c->Lpad, c->Q, c->P, c->alpha, c->loci_per_range, c->delta_part, (const uint32_t *)c->repd);
         }
         LAUNCH_OK();
         hipLaunchKernelGGL(k_metro_accept, dim3(c->q_blocks), dim3(kNT), 0, c->stream, c->ma,
                            c->n_ranges, c->delta_part, c->alpha, c->Q, c->QSum, (int)collect,
-                           c->logq_part, (int32_t *)nullptr, rep);
+                           c->logq_part, (int32_t *)nullptr, (const uint32_t *)c->repd);
+        c->logq_rows = c->q_blocks;
     } else {
-        hipLaunchKernelGGL(k_update_q_gibbs, dim3(c->q_blocks), dim3(kNT), 0, c->stream, c->ma,
-                           c->nQpart, c->n_lt, c->alpha, c->Q, c->QSum, (int)collect, c->logq_part,
-                           rep);
+        const size_t nk = (size_t)c->m.N * c->m.K;
+        hipLaunchKernelGGL(k_fold_rows, dim3(cdiv(nk, kNT), kRowSplit), dim3(kNT), 0, c->stream,
+                           c->nQpart, c->n_lt, nk, c->nQfold);
+        LAUNCH_OK();
+        hipLaunchKernelGGL(k_update_q_gibbs, dim3(cdiv(c->m.N, kGibbsInds)), dim3(kGibbsInds), 0,
+                           c->stream, c->ma, c->nQfold, c->alpha, c->Q, c->QSum, (int)collect,
+                           c->logq_part, (const uint32_t *)c->repd);
+        c->logq_rows = (int)cdiv(c->m.N, kGibbsInds);
     }
     LAUNCH_OK();
     return SGPU_OK;
@@ -706,7 +792,7 @@ int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
 int sgpu_update_z(sgpu_ctx *c, uint32_t rep, int32_t flags, int8_t *zout)
 {
     int rc = check_ready(c);
-    if (rc) {
+    if (rc || (rc = use_rep(c, rep))) {
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_Z);
@@ -733,12 +819,12 @@ int sgpu_update_z(sgpu_ctx *c, uint32_t rep, int32_t flags, int8_t *zout)
 int sgpu_update_alpha(sgpu_ctx *c, uint32_t rep)
 {
     int rc = check_ready(c);
-    if (rc) {
+    if (rc || (rc = use_rep(c, rep))) {
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_ALPHA);
     hipLaunchKernelGGL(k_update_alpha, dim3(1), dim3(kNT), 0, c->stream, c->ma, c->logq_part,
-                       c->q_blocks, c->alpha, rep);
+                       c->logq_rows, c->alpha, (const uint32_t *)c->repd);
     LAUNCH_OK();
     return SGPU_OK;
 }
@@ -746,7 +832,7 @@ int sgpu_update_alpha(sgpu_ctx *c, uint32_t rep)
 int sgpu_update_lambda(sgpu_ctx *c, uint32_t rep)
 {
     int rc = check_ready(c);
-    if (rc) {
+    if (rc || (rc = use_rep(c, rep))) {
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_LAMBDA);
@@ -754,7 +840,7 @@ int sgpu_update_lambda(sgpu_ctx *c, uint32_t rep)
                        c->P, c->eps, c->lam_part);
     LAUNCH_OK();
     hipLaunchKernelGGL(k_lambda_finalize, dim3(1), dim3(kNT), 0, c->stream, c->ma, c->lam_part,
-                       c->l_blocks, c->ahist, c->nhist, c->lambda, rep);
+                       c->l_blocks, c->ahist, c->nhist, c->lambda, (const uint32_t *)c->repd);
     LAUNCH_OK();
     return SGPU_OK;
 }
@@ -762,12 +848,12 @@ int sgpu_update_lambda(sgpu_ctx *c, uint32_t rep)
 int sgpu_update_eps(sgpu_ctx *c, uint32_t rep, int32_t collect)
 {
     int rc = check_ready(c);
-    if (rc) {
+    if (rc || (rc = use_rep(c, rep))) {
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_EPS);
     hipLaunchKernelGGL(k_update_eps, dim3(c->l_blocks), dim3(kNT), 0, c->stream, c->ma, c->nall, c->P,
-                       c->F, c->lambda, c->eps, c->epsSum, (int)collect, rep);
+                       c->F, c->lambda, c->eps, c->epsSum, (int)collect, (const uint32_t *)c->repd);
     LAUNCH_OK();
     return SGPU_OK;
 }
@@ -775,15 +861,15 @@ int sgpu_update_eps(sgpu_ctx *c, uint32_t rep, int32_t collect)
 int sgpu_update_fst(sgpu_ctx *c, uint32_t rep)
 {
     int rc = check_ready(c);
-    if (rc) {
+    if (rc || (rc = use_rep(c, rep))) {
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_FST);
     hipLaunchKernelGGL(k_fst_partial, dim3(c->l_blocks), dim3(kNT), 0, c->stream, c->ma, c->nall,
-                       c->eps, c->P, c->F, c->fst_part, rep);
+                       c->eps, c->P, c->F, c->fst_part, (const uint32_t *)c->repd);
     LAUNCH_OK();
     hipLaunchKernelGGL(k_fst_finalize, dim3(1), dim3(kNT), 0, c->stream, c->ma, c->fst_part,
-                       c->l_blocks, c->nloci_nonzero, c->F, rep);
+                       c->l_blocks, c->nloci_nonzero, c->F, (const uint32_t *)c->repd);
     LAUNCH_OK();
     return SGPU_OK;
 }
@@ -829,11 +915,66 @@ int sgpu_sweep(sgpu_ctx *c, uint32_t rep)
 
 int sgpu_run_sweeps(sgpu_ctx *c, uint32_t rep0, uint32_t n)
 {
-    for (uint32_t r = 0; r < n; r++) {
-        int rc = sgpu_sweep(c, rep0 + r);
-        if (rc) {
-            return rc;
+    int rc = check_ready(c);
+    if (rc) {
+        return rc;
+    }
+    if (!c->use_graphs || c->timing) {
+        for (uint32_t r = 0; r < n; r++) {
+            if ((rc = sgpu_sweep(c, rep0 + r))) {
+                return rc;
+            }
         }
+        return SGPU_OK;
+    }
+    // One graph per sweep type (Metropolis or Gibbs Q, burn-in or collecting),
+    // captured on first use; each replay ends by advancing the device index.
+    const sgpu_model &m = c->m;
+    if ((rc = use_rep(c, rep0))) {
+        return rc;
+    }
+    for (uint32_t r = 0; r < n; r++) {
+        const uint32_t rep = rep0 + r;
+        const bool metro = m.metrofreq > 0 && rep % (uint32_t)m.metrofreq == 0;
+        const bool collect = (int64_t)rep + 1 > m.burnin;
+        const int type = (metro ? 1 : 0) | (collect ? 2 : 0);
+        if (!c->graph[type]) {
+            hipGraph_t g = nullptr;
+            HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            c->capturing = true;
+            rc = sgpu_sweep(c, rep);
+            if (rc == SGPU_OK) {
+                hipLaunchKernelGGL(k_advance_rep, dim3(1), dim3(1), 0, c->stream, c->repd);
+            }
+            c->capturing = false;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc) {
+                if (g) {
+                    (void)hipGraphDestroy(g);
+                }
+                return rc;
+            }
+            if (e != hipSuccess) {
+                return fail(SGPU_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+            }
+            e = hipGraphInstantiate(&c->graph[type], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                c->graph[type] = nullptr;
+                return fail(SGPU_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
+            }
+            // the captured sweep counted itself; the replay below counts again
+            if (collect) {
+                c->nsum--;
+            }
+        }
+        HIP_OK(hipGraphLaunch(c->graph[type], c->stream));
+        c->rep_shadow = rep + 1;
+        if (collect) {
+            c->nsum++;
+        }
+        c->like_ready = collect && m.computeprob;
+        c->logq_rows = metro ? c->q_blocks : (int)cdiv(m.N, kGibbsInds);
     }
     return SGPU_OK;
 }

@@ -39,7 +39,7 @@ struct ZArgs {
     int N, L, K, A, Lpad, G;   // G = number of genotype word rows
     int rows_per_tile;         // word rows per individual tile
     int inds_per_row;          // 16 (SNP2) or 2 (BYTE)
-    uint32_t seed, chain, rep;
+    uint32_t seed, chain;
 };
 
 }  // namespace sgpu

@@ -272,3 +272,24 @@ def test_chain_trajectory_matches_oracle(kind):
     like = gch.get(sa.LIKE)
     _close(like, [och.like, och.sumlikes, och.sumsqlikes], 1e-9)
     assert gch.get(sa.NSUM)[0] == och.state.nsum == 23
+
+
+def test_graph_replays_match_direct_launches():
+    """sgpu_run_sweeps replays captured HIP graphs (one per sweep type) with the
+    sweep index read on the device; every replay block must equal the same
+    sweeps launched one by one, with host reads and syncs in between."""
+    geno, nall, A = _snp(80, 700, 4, seed=8, miss=0.02)
+    kw = dict(freqscorr=1, inferalpha=1, inferlambda=1, burnin=9, metrofreq=5, seed=31)
+    ga = sa.Chain.from_geno(sa.make_model(80, 700, 4, A, **kw), geno, nall)
+    gb = sa.Chain.from_geno(sa.make_model(80, 700, 4, A, **kw), geno, nall)
+    ga.init()
+    gb.init()
+    rep = 0
+    for block in (7, 7, 6, 11):
+        ga.run(rep, block)
+        for r in range(rep, rep + block):
+            gb.sweep(r)
+        rep += block
+        for sel in (sa.Q, sa.P, sa.EPS, sa.FST, sa.ALPHA, sa.LAMBDA, sa.NA, sa.NQ, sa.QSUM,
+                    sa.PSUM, sa.LIKE, sa.NSUM):
+            assert np.array_equal(ga.get(sel), gb.get(sel)), (block, sel)
