@@ -573,6 +573,8 @@ __global__ __launch_bounds__(kNT) void k_lambda_finalize(ModelArgs m, const doub
 // Epsilon (NonIndependenceUpdateEpsilon, UpdateEpsilon.c:71-145): one thread
 // per locus.  Fused: SumEpsilon += eps (output.c:684-687) when collecting.
 // ---------------------------------------------------------------------------
+// One thread per (locus, pop) term; a block holds whole loci (K * (256 / K)
+// threads) so the K terms of a locus meet in LDS.
 __global__ __launch_bounds__(kNT) void k_update_eps(ModelArgs m, const int32_t *__restrict__ nall,
                                                      const double *__restrict__ P,
                                                      const double *__restrict__ F,
@@ -582,32 +584,52 @@ __global__ __launch_bounds__(kNT) void k_update_eps(ModelArgs m, const int32_t *
                                                      const uint32_t *__restrict__ repp)
 {
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
-    const int l = blockIdx.x * kNT + threadIdx.x;
-    if (l >= m.L) {
-        return;
-    }
-    const int na = nall[l];
-    double *e = eps + (size_t)l * m.A;
+    __shared__ double s_term[6][kNT];   // the six terms of each (locus, pop), summed in order below
+    const int K = m.K;
+    const int lpb = kNT / K;                       // loci per block
+    const int j = threadIdx.x / K, k = threadIdx.x % K;
+    const int l = blockIdx.x * lpb + j;
+    const bool live = j < lpb && l < m.L;
+    const int na = live ? nall[l] : 0;
+    double *e = eps + (size_t)(live ? l : 0) * m.A;
+    int a1 = 0, a2 = 0;
+    double d = 0.0, e1 = 0.0, e2 = 0.0;
+    bool inrange = false;
+    u32x4 w{0u, 0u, 0u, 0u};
     if (na > 1) {
-        const u32x4 w = philox4x32_10(rep, (uint32_t)l, 0u, PH_EPS << 24, m.seed, m.chain);
-        const int a1 = (int)(uz(w.x) * (double)na);
-        int a2 = (int)(uz(w.y) * (double)(na - 1));
+        w = philox4x32_10(rep, (uint32_t)l, 0u, PH_EPS << 24, m.seed, m.chain);
+        a1 = (int)(uz(w.x) * (double)na);
+        a2 = (int)(uz(w.y) * (double)(na - 1));
         if (a2 >= a1) {
             a2 += 1;
         }
         const double invsqrtn = pow((double)m.N, -0.5);
-        const double d = 0.0 + uz(w.z) * (invsqrtn - 0.0);
-        const double e1 = e[a1], e2 = e[a2];
-        if ((e1 + d < 1.0) && (e2 - d > 0.0)) {
+        d = 0.0 + uz(w.z) * (invsqrtn - 0.0);
+        e1 = e[a1];
+        e2 = e[a2];
+        inrange = (e1 + d < 1.0) && (e2 - d > 0.0);
+    }
+    if (inrange) {
+        const double frac = (1.0 - F[k]) / F[k];
+        s_term[0][threadIdx.x] = lgamma_lanczos(frac * e1);
+        s_term[1][threadIdx.x] = lgamma_lanczos(frac * e2);
+        s_term[2][threadIdx.x] = lgamma_lanczos(frac * (e1 + d));
+        s_term[3][threadIdx.x] = lgamma_lanczos(frac * (e2 - d));
+        s_term[4][threadIdx.x] = frac * d * log(P[((size_t)l * K + k) * m.A + a1]);
+        s_term[5][threadIdx.x] = frac * d * log(P[((size_t)l * K + k) * m.A + a2]);
+    }
+    __syncthreads();
+    if (live && k == 0) {
+        if (inrange) {
             double sum = 0.0;
-            for (int k = 0; k < m.K; k++) {
-                const double frac = (1.0 - F[k]) / F[k];
-                sum += lgamma_lanczos(frac * e1);
-                sum += lgamma_lanczos(frac * e2);
-                sum -= lgamma_lanczos(frac * (e1 + d));
-                sum -= lgamma_lanczos(frac * (e2 - d));
-                sum += frac * d * log(P[((size_t)l * m.K + k) * m.A + a1]);
-                sum -= frac * d * log(P[((size_t)l * m.K + k) * m.A + a2]);
+            for (int kk = 0; kk < K; kk++) {   // the reference's accumulation order
+                const int t = threadIdx.x + kk;
+                sum += s_term[0][t];
+                sum += s_term[1][t];
+                sum -= s_term[2][t];
+                sum -= s_term[3][t];
+                sum += s_term[4][t];
+                sum -= s_term[5][t];
             }
             const double lam = lambda[0];
             if (lam != 1.0) {
@@ -618,10 +640,10 @@ __global__ __launch_bounds__(kNT) void k_update_eps(ModelArgs m, const int32_t *
                 e[a2] = e2 - d;
             }
         }
-    }
-    if (collect) {
-        for (int aa = 0; aa < na; aa++) {
-            epsSum[(size_t)l * m.A + aa] += e[aa];
+        if (collect) {
+            for (int aa = 0; aa < na; aa++) {
+                epsSum[(size_t)l * m.A + aa] += e[aa];
+            }
         }
     }
 }
@@ -647,16 +669,23 @@ __global__ __launch_bounds__(kNT) void k_fst_partial(ModelArgs m, const int32_t 
                                                       const double *__restrict__ eps,
                                                       const double *__restrict__ P,
                                                       const double *__restrict__ F,
-                                                      double *__restrict__ part, const uint32_t *__restrict__ repp)
+                                                      double *__restrict__ part,
+                                                      const uint32_t *__restrict__ repp)
 {
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     __shared__ double s_red[kNT / 64];
+    __shared__ double s_new[kMaxK], s_old[kMaxK];
+    const int numpops1 = m.onefst ? 1 : m.K;
+    if (threadIdx.x < numpops1) {   // the block's proposals, drawn once
+        s_old[threadIdx.x] = F[threadIdx.x];
+        s_new[threadIdx.x] = fst_proposal(m, s_old[threadIdx.x], threadIdx.x, rep, nullptr);
+    }
+    __syncthreads();
     const int l = blockIdx.x * kNT + threadIdx.x;
     const int na = (l < m.L) ? nall[l] : 0;
-    const int numpops1 = m.onefst ? 1 : m.K;
     for (int p1 = 0; p1 < numpops1; p1++) {
-        const double oldf = F[p1];
-        const double newf = fst_proposal(m, oldf, p1, rep, nullptr);
+        const double oldf = s_old[p1];
+        const double newf = s_new[p1];
         double v = 0.0;
         if (newf > 0.0 && newf < 1.0 && na > 0) {
             const double nfr = (1.0 - newf) / newf;
@@ -680,7 +709,8 @@ __global__ __launch_bounds__(kNT) void k_fst_partial(ModelArgs m, const int32_t 
 
 __global__ __launch_bounds__(kNT) void k_fst_finalize(ModelArgs m, const double *__restrict__ part,
                                                        int nparts, int nloci_nonzero,
-                                                       double *__restrict__ F, const uint32_t *__restrict__ repp)
+                                                       double *__restrict__ F,
+                                                       const uint32_t *__restrict__ repp)
 {
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     __shared__ double s_red[kNT / 64];
@@ -696,29 +726,29 @@ __global__ __launch_bounds__(kNT) void k_fst_finalize(ModelArgs m, const double 
             s_stat[p1] = t;
         }
     }
-    if (threadIdx.x != 0) {
+    __syncthreads();
+    // the proposal groups are independent (each reads and writes its own F), so
+    // thread p1 decides group p1
+    const int p1 = threadIdx.x;
+    if (p1 >= numpops1) {
         return;
     }
     const double mu = m.fpriormean, sd = m.fpriorsd;
-    for (int p1 = 0; p1 < numpops1; p1++) {
-        uint32_t uw;
-        const double oldf = F[p1];
-        const double newf = fst_proposal(m, oldf, p1, rep, &uw);
-        if (newf > 0.0 && newf < 1.0) {
-            double lpd = ((mu * mu / (sd * sd) - 1) * log(newf / oldf) +
-                          (oldf - newf) * mu / (sd * sd));
-            const double nfr = (1.0 - newf) / newf;
-            const double ofr = (1.0 - oldf) / oldf;
-            const int ngroup = m.onefst ? m.K : 1;
-            lpd += (double)ngroup * (double)nloci_nonzero *
-                   (lgamma_lanczos(nfr) - lgamma_lanczos(ofr));
-            lpd += s_stat[p1];
-            const bool acc = (lpd >= 0.0) || (uo(uw) < exp(lpd));
-            if (acc) {
-                const int p2end = m.onefst ? m.K : p1 + 1;
-                for (int p2 = p1; p2 < p2end; p2++) {
-                    F[p2] = newf;
-                }
+    uint32_t uw;
+    const double oldf = F[p1];
+    const double newf = fst_proposal(m, oldf, p1, rep, &uw);
+    if (newf > 0.0 && newf < 1.0) {
+        double lpd = ((mu * mu / (sd * sd) - 1) * log(newf / oldf) + (oldf - newf) * mu / (sd * sd));
+        const double nfr = (1.0 - newf) / newf;
+        const double ofr = (1.0 - oldf) / oldf;
+        const int ngroup = m.onefst ? m.K : 1;
+        lpd += (double)ngroup * (double)nloci_nonzero * (lgamma_lanczos(nfr) - lgamma_lanczos(ofr));
+        lpd += s_stat[p1];
+        const bool acc = (lpd >= 0.0) || (uo(uw) < exp(lpd));
+        if (acc) {
+            const int p2end = m.onefst ? m.K : p1 + 1;
+            for (int p2 = p1; p2 < p2end; p2++) {
+                F[p2] = newf;
             }
         }
     }

@@ -51,10 +51,11 @@ __device__ __forceinline__ uint32_t add_lane_bit(uint32_t v, uint64_t mask)
     return r;
 }
 
-// Parks a wave-uniform value in lane lane_idx of a collector register.
-__device__ __forceinline__ uint32_t write_lane(uint32_t v, uint32_t value, int lane_idx)
+// v += value (wave-uniform) in the lanes that execute it; asm keeps the
+// compiler from turning the single-lane branch around it into a select.
+__device__ __forceinline__ void collect_add(uint32_t &v, uint32_t value)
 {
-    return ((int)(threadIdx.x & 63) == lane_idx) ? value : v;
+    asm volatile("v_add_u32 %0, %0, %1" : "+v"(v) : "s"(value));
 }
 
 // Running product of clamped terms with exponent bookkeeping (CalcLikeInd's
@@ -81,8 +82,8 @@ struct LikeAcc {
 //   per-locus counters   T[k]  = #copies with z > k, A1[k] = #allele-1 copies
 //                        with z > k (lane-private registers, carry-in adds);
 //   per-individual       popcount of the wave ballot of each predicate (SALU),
-//                        parked in lane t of a collector register and added
-//                        into LDS once per 16 individuals.
+//                        added into lane t of a collector register by that lane
+//                        alone, and into LDS once per 16 individuals.
 // Counts per population are differences of consecutive "z > k" counts.
 // Q and P reach the kernel through restrict pointers, so Q[i,:] (wave
 // uniform) is fetched with scalar loads.
@@ -165,7 +166,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
                 Ttot += valid;
                 A1tot += valid && x0;
                 A1tot += valid && x1;
-                col[0] = write_lane(col[0], 2u * (uint32_t)__popcll(__ballot(valid)), t);
+                uint32_t nk[K];   // this individual's counts (wave-uniform)
+                nk[0] = 2u * (uint32_t)__popcll(__ballot(valid));
                 int z0 = 0, z1 = 0;
                 if (INIT) {
                     const uint32_t u0 = (uint32_t)(((uint64_t)wa * (uint32_t)K) >> 32);
@@ -178,8 +180,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
                         T[k] += m1;
                         A1[k] += m0 && x0;
                         A1[k] += m1 && x1;
-                        const uint32_t n = (uint32_t)(__popcll(__ballot(m0)) + __popcll(__ballot(m1)));
-                        col[k + 1] = write_lane(col[k + 1], n, t);
+                        nk[k + 1] = (uint32_t)(__popcll(__ballot(m0)) + __popcll(__ballot(m1)));
                     }
                 } else {
                     double pre0[K], pre1[K];
@@ -216,8 +217,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
                         const uint64_t m1 = (xm1 & b1) | (~xm1 & a1);   // z1 > k
                         T[k] = add_lane_bit(add_lane_bit(T[k], m0), m1);
                         A1[k] = add_lane_bit(add_lane_bit(A1[k], m0 & xm0), m1 & xm1);
-                        const uint32_t n = (uint32_t)(__popcll(m0) + __popcll(m1));
-                        col[k + 1] = write_lane(col[k + 1], n, t);
+                        nk[k + 1] = (uint32_t)(__popcll(m0) + __popcll(m1));
                         if (ZOUT) {
                             z0 += (int)((m0 >> lane) & 1u);
                             z1 += (int)((m1 >> lane) & 1u);
@@ -232,14 +232,21 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
                         T[k] += m1;
                         A1[k] += m0 && x0;
                         A1[k] += m1 && x1;
-                        const uint32_t n = (uint32_t)(__popcll(__ballot(m0)) + __popcll(__ballot(m1)));
-                        col[k + 1] = write_lane(col[k + 1], n, t);
+                        nk[k + 1] = (uint32_t)(__popcll(__ballot(m0)) + __popcll(__ballot(m1)));
                         if (ZOUT) {
                             z0 += m0;
                             z1 += m1;
                         }
                     }
 #endif
+                }
+                // park the counts in lane t of the collector: one add per value,
+                // executed by that lane only
+                if (lane == t) {
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        collect_add(col[k], nk[k]);
+                    }
                 }
                 if (ZOUT && lok) {
                     zout[((size_t)i * 2 + 0) * d.L + l] = valid ? (int8_t)z0 : (int8_t)-9;

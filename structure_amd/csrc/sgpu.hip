@@ -852,7 +852,8 @@ int sgpu_update_eps(sgpu_ctx *c, uint32_t rep, int32_t collect)
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_EPS);
-    hipLaunchKernelGGL(k_update_eps, dim3(c->l_blocks), dim3(kNT), 0, c->stream, c->ma, c->nall, c->P,
+    hipLaunchKernelGGL(k_update_eps, dim3(cdiv(c->m.L, kNT / c->m.K)), dim3(kNT), 0, c->stream,
+                       c->ma, c->nall, c->P,
                        c->F, c->lambda, c->eps, c->epsSum, (int)collect, (const uint32_t *)c->repd);
     LAUNCH_OK();
     return SGPU_OK;
