@@ -63,6 +63,46 @@ def cpu_baseline(N, L, K, sample_loci, sweeps, seed):
                        f"{dt:.1f} s, scaled linearly to L={L}")}
 
 
+def batched_ksweep(d, N, L, seed, sweeps, warmup, kmax=10, reps=10):
+    """The K-sweep users run (K = 1..kmax, reps Philox seeds each) on one data
+    set: chain-sweeps/s with all chains in one batch (one launch per kernel and
+    phase, HIP-graph replay) against the same chains run one at a time on the
+    GPU (each alone in its own context, graph replay)."""
+    import structure_amd as sa
+    models = [sa.make_model(N, L, K, 2, freqscorr=1, inferalpha=1, inferlambda=1, metrofreq=10,
+                            burnin=warmup, numreps=sweeps, seed=seed + r, chain=K)
+              for K in range(1, kmax + 1) for r in range(reps)]
+    data = [(sa.GENO_SNP2, d["words"], d["nall"])]
+    b = sa.Batch(models, data)
+    b.init()
+    b.run(0, warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(warmup, sweeps)
+    b.synchronize()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    t_plain = 0.0
+    for m in models:
+        ch = sa.Chain(m, sa.GENO_SNP2, d["words"], d["nall"])
+        ch.init()
+        ch.run(0, warmup)
+        ch.synchronize()
+        t0 = time.perf_counter()
+        ch.run(warmup, sweeps)
+        ch.synchronize()
+        t_plain += time.perf_counter() - t0
+        ch.close()
+    n = len(models)
+    return {"chains": n, "K": f"1..{kmax} x {reps} seeds", "sweeps_timed": sweeps,
+            "warmup": warmup,
+            "batched_chain_sweeps_per_s": round(n * sweeps / t_batch, 2),
+            "plain_chain_sweeps_per_s": round(n * sweeps / t_plain, 2),
+            "speedup": round(t_plain / t_batch, 3),
+            "batched_site_updates_per_s": round(
+                sum(N * L * m.K for m in models) * sweeps / t_batch, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +115,8 @@ def main():
     ap.add_argument("--cpu-sample-loci", type=int, default=10000)
     ap.add_argument("--cpu-sweeps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-sweeps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
@@ -152,6 +194,9 @@ def main():
         "phase_timing_sweeps": nt,
         "lnP_last": like[0],
     }
+    if not args.no_batch:
+        print("bench: batched K-sweep (100 chains) ...", file=sys.stderr, flush=True)
+        result["batched"] = batched_ksweep(d, N, L, args.seed, args.batch_sweeps, 10)
     if not args.no_cpu:
         print("bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(N, L, K, args.cpu_sample_loci, args.cpu_sweeps,

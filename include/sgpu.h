@@ -111,6 +111,33 @@ int sgpu_pack_byte(const int8_t *geno, int32_t N, int32_t L, uint32_t *out, size
  * allocates device state and uploads the packed genotypes. */
 int sgpu_create(const sgpu_model *m, const sgpu_data *d, sgpu_ctx **out);
 
+/* Tile targets of the batched launches (0 = library default, about 2048
+ * blocks, 8 per CU): UpdateZ individual tiles and Metropolis locus ranges are
+ * chosen so that one launch over the whole batch has about this many blocks.
+ * The counts do not depend on them; lnP(D) and the Metropolis log-ratios are
+ * FP64 sums whose grouping does (differences in the last bits). */
+typedef struct {
+    int32_t z_blocks;
+    int32_t metro_blocks;
+} sgpu_tuning;
+
+/* A batch of chains in one context (batched form of InitCLDict; the reference
+ * runs one chain per process).  Chain c uses models[c] and data set
+ * data[data_of_chain[c]]; chains sharing a data set must agree on N, L, A, and
+ * all chains on BURNIN, METROFREQ and COMPUTEPROB.  K, the flags, the priors
+ * and the Philox key (seed, chain) may differ per chain.  Every per-phase call
+ * and sweep below then advances all chains with one launch per kernel (per
+ * distinct (format, K) for UpdateZ and the Metropolis ratio).  Each chain's
+ * random stream depends only on its own key, so a chain's Z, counts, P, Q,
+ * alpha, lambda, eps and F are those of the same chain run alone.
+ * tuning may be NULL. */
+int sgpu_create_batch(const sgpu_model *models, int32_t nchains, const sgpu_data *data,
+                      int32_t ndata, const int32_t *data_of_chain, const sgpu_tuning *tuning,
+                      sgpu_ctx **out);
+
+/* Number of chains in the context. */
+int sgpu_num_chains(const sgpu_ctx *ctx);
+
 /* Replaces ReleaseCLDict (Kernels.c:177-193). */
 int sgpu_destroy(sgpu_ctx *ctx);
 
@@ -124,7 +151,15 @@ int sgpu_init_state(sgpu_ctx *ctx);
 int sgpu_get_state(sgpu_ctx *ctx, int32_t which, void *host, size_t bytes);
 int sgpu_set_state(sgpu_ctx *ctx, int32_t which, const void *host, size_t bytes);
 
+/* The same for chain `chain` of a batch (sgpu_get_state reads chain 0). */
+int sgpu_get_chain_state(sgpu_ctx *ctx, int32_t chain, int32_t which, void *host, size_t bytes);
+int sgpu_set_chain_state(sgpu_ctx *ctx, int32_t chain, int32_t which, const void *host,
+                         size_t bytes);
+
 /* ---- per-phase updates (asynchronous on the context stream) --------------- */
+/* Each call advances every chain of the context.  The alpha, lambda, epsilon
+ * and Fst updates act on the chains whose model enables them (INFERALPHA,
+ * INFERLAMBDA, FREQSCORR), as structure.c:581-605 guards their calls. */
 
 /* UpdatePCL (UpdateP.c:202-273, UpdateP.h:4-7): P[l,k,:] ~ Dirichlet from the
  * allele counts of the last UpdateZ.  collect != 0 also adds P into PSum. */

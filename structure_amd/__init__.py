@@ -48,6 +48,13 @@ class Data(C.Structure):
                 ("geno_words", C.c_size_t), ("nall", C.POINTER(C.c_int32))]
 
 
+class Tuning(C.Structure):
+    """Mirror of ``sgpu_tuning``: block targets of the batched UpdateZ and
+    Metropolis launches (0 = library default)."""
+
+    _fields_ = [("z_blocks", C.c_int32), ("metro_blocks", C.c_int32)]
+
+
 # Defaults of the shipped extraparams / PresetValues (extraparams:1-120, params.c:571-670)
 DEFAULTS = dict(burnin=0, numreps=0, freqscorr=1, onefst=0, inferalpha=1, popalphas=0,
                 unifprioralpha=1, inferlambda=0, popspecificlambda=0, metrofreq=10,
@@ -75,6 +82,11 @@ def lib() -> C.CDLL:
             "sgpu_pack_snp2": (C.c_int, [vp, i32, i32, vp, sz]),
             "sgpu_pack_byte": (C.c_int, [vp, i32, i32, vp, sz]),
             "sgpu_create": (C.c_int, [C.POINTER(Model), C.POINTER(Data), C.POINTER(vp)]),
+            "sgpu_create_batch": (C.c_int, [C.POINTER(Model), i32, C.POINTER(Data), i32, vp,
+                                            C.POINTER(Tuning), C.POINTER(vp)]),
+            "sgpu_num_chains": (C.c_int, [vp]),
+            "sgpu_get_chain_state": (C.c_int, [vp, i32, i32, vp, sz]),
+            "sgpu_set_chain_state": (C.c_int, [vp, i32, i32, vp, sz]),
             "sgpu_destroy": (C.c_int, [vp]),
             "sgpu_init_state": (C.c_int, [vp]),
             "sgpu_get_state": (C.c_int, [vp, i32, vp, sz]),
@@ -167,31 +179,23 @@ def synth_microsat(N, L, amin=2, amax=10, Ktrue=3, qalpha=0.5, miss=0.02, seed=1
 
 # ---- chain context ----------------------------------------------------------
 
-class Chain:
-    """One MCMC chain on the GPU (an ``sgpu_ctx``).
+def _state_shape(m: Model, which):
+    N, L, K, A = m.N, m.L, m.K, m.A
+    return {Q: ((N, K), np.float64), P: ((L, K, A), np.float64), EPS: ((L, A), np.float64),
+            FST: ((K,), np.float64), ALPHA: ((K,), np.float64), LAMBDA: ((K,), np.float64),
+            NA: ((L, K, A), np.int32), NQ: ((N, K), np.int32), QSUM: ((N, K), np.float64),
+            PSUM: ((L, K, A), np.float64), FSTSUM: ((K,), np.float64),
+            EPSSUM: ((L, A), np.float64), ALPHASUM: ((K,), np.float64),
+            LAMBDASUM: ((K,), np.float64), LIKE: ((3,), np.float64),
+            NSUM: ((1,), np.int64)}[which]
 
-    Method names follow the reference's host wrappers: ``update_p`` ~ UpdatePCL,
-    ``update_q`` ~ UpdateQCL, ``update_z`` ~ UpdateZCL, ``update_alpha`` ~
-    UpdateAlphaCL, ``update_lambda`` ~ UpdateLambda, ``update_eps`` ~
-    UpdateEpsilonCL, ``update_fst`` ~ UpdateFstCL, ``collect`` ~ DataCollectionCL;
-    ``get``/``set`` ~ readBuffer/writeBuffer."""
 
-    def __init__(self, model: Model, fmt: int, words: np.ndarray, nall: np.ndarray):
-        self.model = model
-        self._words = np.ascontiguousarray(words, dtype=np.uint32)
-        self._nall = np.ascontiguousarray(nall, dtype=np.int32)
-        data = Data(format=fmt, geno=self._words.ctypes.data_as(C.POINTER(C.c_uint32)),
-                    geno_words=self._words.size,
-                    nall=self._nall.ctypes.data_as(C.POINTER(C.c_int32)))
-        h = C.c_void_p()
-        _check(lib().sgpu_create(C.byref(model), C.byref(data), C.byref(h)))
-        self._h = h
-        self.fmt = fmt
+class _Context:
+    """Phase entry points shared by single chains and batches.  Each call
+    advances every chain of the context."""
 
-    @classmethod
-    def from_geno(cls, model: Model, geno: np.ndarray, nall: np.ndarray, fmt=None):
-        f, words = pack(geno, fmt)
-        return cls(model, f, words, nall)
+    _h = None
+    models: list
 
     def close(self):
         if getattr(self, "_h", None):
@@ -210,28 +214,17 @@ class Chain:
     def __exit__(self, *a):
         self.close()
 
-    # state
-    def _shape(self, which):
-        m = self.model
-        N, L, K, A = m.N, m.L, m.K, m.A
-        return {Q: ((N, K), np.float64), P: ((L, K, A), np.float64), EPS: ((L, A), np.float64),
-                FST: ((K,), np.float64), ALPHA: ((K,), np.float64), LAMBDA: ((K,), np.float64),
-                NA: ((L, K, A), np.int32), NQ: ((N, K), np.int32), QSUM: ((N, K), np.float64),
-                PSUM: ((L, K, A), np.float64), FSTSUM: ((K,), np.float64),
-                EPSSUM: ((L, A), np.float64), ALPHASUM: ((K,), np.float64),
-                LAMBDASUM: ((K,), np.float64), LIKE: ((3,), np.float64),
-                NSUM: ((1,), np.int64)}[which]
-
-    def get(self, which) -> np.ndarray:
-        shape, dt = self._shape(which)
+    # state of one chain
+    def get_chain(self, chain, which) -> np.ndarray:
+        shape, dt = _state_shape(self.models[chain], which)
         out = np.empty(shape, dt)
-        _check(lib().sgpu_get_state(self._h, which, _vp(out), out.nbytes))
+        _check(lib().sgpu_get_chain_state(self._h, chain, which, _vp(out), out.nbytes))
         return out
 
-    def set(self, which, value):
-        shape, dt = self._shape(which)
+    def set_chain(self, chain, which, value):
+        shape, dt = _state_shape(self.models[chain], which)
         a = np.ascontiguousarray(np.broadcast_to(np.asarray(value, dtype=dt), shape))
-        _check(lib().sgpu_set_state(self._h, which, _vp(a), a.nbytes))
+        _check(lib().sgpu_set_chain_state(self._h, chain, which, _vp(a), a.nbytes))
 
     # phases
     def init(self):
@@ -244,7 +237,7 @@ class Chain:
         _check(lib().sgpu_update_q(self._h, rep, int(collect)))
 
     def update_z(self, rep, like=False, want_z=False):
-        m = self.model
+        m = self.models[0]
         z = np.empty((m.N, 2, m.L), np.int8) if want_z else None
         _check(lib().sgpu_update_z(self._h, rep, Z_LIKE if like else 0,
                                    _vp(z) if want_z else None))
@@ -278,8 +271,75 @@ class Chain:
         _check(lib().sgpu_set_timing(self._h, int(enable)))
 
     def get_timing(self):
-        """{phase: (total ms, launches)} since the last call (HIP events on the chain's stream)."""
+        """{phase: (total ms, launches)} since the last call (HIP events on the context stream)."""
         ms = np.zeros(len(PHASES))
         n = np.zeros(len(PHASES), np.int64)
         _check(lib().sgpu_get_timing(self._h, _vp(ms), _vp(n)))
         return {p: (float(ms[i]), int(n[i])) for i, p in enumerate(PHASES)}
+
+
+class Chain(_Context):
+    """One MCMC chain on the GPU (an ``sgpu_ctx``).
+
+    Method names follow the reference's host wrappers: ``update_p`` ~ UpdatePCL,
+    ``update_q`` ~ UpdateQCL, ``update_z`` ~ UpdateZCL, ``update_alpha`` ~
+    UpdateAlphaCL, ``update_lambda`` ~ UpdateLambda, ``update_eps`` ~
+    UpdateEpsilonCL, ``update_fst`` ~ UpdateFstCL, ``collect`` ~ DataCollectionCL;
+    ``get``/``set`` ~ readBuffer/writeBuffer."""
+
+    def __init__(self, model: Model, fmt: int, words: np.ndarray, nall: np.ndarray):
+        self.model = model
+        self.models = [model]
+        self._words = np.ascontiguousarray(words, dtype=np.uint32)
+        self._nall = np.ascontiguousarray(nall, dtype=np.int32)
+        data = Data(format=fmt, geno=self._words.ctypes.data_as(C.POINTER(C.c_uint32)),
+                    geno_words=self._words.size,
+                    nall=self._nall.ctypes.data_as(C.POINTER(C.c_int32)))
+        h = C.c_void_p()
+        _check(lib().sgpu_create(C.byref(model), C.byref(data), C.byref(h)))
+        self._h = h
+        self.fmt = fmt
+
+    @classmethod
+    def from_geno(cls, model: Model, geno: np.ndarray, nall: np.ndarray, fmt=None):
+        f, words = pack(geno, fmt)
+        return cls(model, f, words, nall)
+
+    def get(self, which) -> np.ndarray:
+        return self.get_chain(0, which)
+
+    def set(self, which, value):
+        self.set_chain(0, which, value)
+
+
+class Batch(_Context):
+    """Several chains advanced together (``sgpu_create_batch``): one launch per
+    kernel and phase over all of them.
+
+    ``datasets`` is a list of ``(fmt, words, nall)``; chain c runs
+    ``models[c]`` on ``datasets[data_of_chain[c]]``."""
+
+    def __init__(self, models, datasets, data_of_chain=None, tuning=None):
+        self.models = list(models)
+        n = len(self.models)
+        if data_of_chain is None:
+            data_of_chain = [0] * n
+        self._dsets = []
+        darr = (Data * len(datasets))()
+        for j, (fmt, words, nall) in enumerate(datasets):
+            w = np.ascontiguousarray(words, dtype=np.uint32)
+            na = np.ascontiguousarray(nall, dtype=np.int32)
+            self._dsets.append((w, na))
+            darr[j] = Data(format=fmt, geno=w.ctypes.data_as(C.POINTER(C.c_uint32)),
+                           geno_words=w.size, nall=na.ctypes.data_as(C.POINTER(C.c_int32)))
+        marr = (Model * n)(*self.models)
+        self._doc = np.ascontiguousarray(data_of_chain, dtype=np.int32)
+        assert self._doc.size == n
+        h = C.c_void_p()
+        t = C.byref(tuning) if tuning is not None else None
+        _check(lib().sgpu_create_batch(marr, n, darr, len(datasets), _vp(self._doc), t,
+                                       C.byref(h)))
+        self._h = h
+
+    def __len__(self):
+        return len(self.models)

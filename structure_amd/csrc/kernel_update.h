@@ -3,6 +3,11 @@
 // Reductions feeding a Metropolis decision are FP64 with a fixed tree
 // (block partials in block order, then one block), so results do not depend
 // on scheduling.
+//
+// Every kernel serves a batch of chains: block b of a launch works on chain
+// map[b].chain at block coordinates (map[b].bx, map[b].by) of that chain's own
+// grid, with the chain's shapes and state addresses taken from cds[chain].
+// A single chain is a batch of one.
 #pragma once
 
 #include "devmath.h"
@@ -12,33 +17,37 @@ namespace sgpu {
 
 constexpr int kNT = 256;   // threads per block for the update kernels
 
-struct ModelArgs {
-    int N, L, K, A;
-    int freqscorr, onefst, popalphas, unifprioralpha, popspecificlambda;
-    double alphapropsd, alphamax, alphapriora, alphapriorb, fpriormean, fpriorsd;
-    uint32_t seed, chain;
-};
+// The chain and block coordinates of this block (uniform: scalar loads).
+#define SGPU_BLOCK_PROLOGUE()                          \
+    const BlockRef br_ = map[blockIdx.x];              \
+    const ChainDesc &C = cds[br_.chain];               \
+    const ModelArgs m = C.m;                           \
+    const int bx = (int)br_.bx;                        \
+    (void)bx
 
 // ---------------------------------------------------------------------------
 // UpdateP (UpdateP.c:106-199): one thread per (locus, pop).  Dirichlet
 // parameters from the allele counts of the last UpdateZ (summed over the
 // individual-tile partials), draw via LogRDirichletDisc's rule
 // (ran.c:1247-1272) including the sum < UNDERFLO uniform fallback.
-// Fused: PSum += P (output.c:673-677) when collecting, and the per-pop sums of
-// log P that UpdatePopLambda / UpdateLambda need without the F model.
+// Fused: PSum += P (output.c:673-677) when collecting.
 // ---------------------------------------------------------------------------
 template <int ASTORE>
-__global__ __launch_bounds__(kNT) void k_update_p(ModelArgs m, const int32_t *__restrict__ nall,
-                                                   const int32_t *__restrict__ nApart, int n_it,
-                                                   const double *__restrict__ eps,
-                                                   const double *__restrict__ F,
-                                                   const double *__restrict__ lambda,
-                                                   double *__restrict__ P,
-                                                   double *__restrict__ PSum, int collect,
-                                                   const uint32_t *__restrict__ repp)
+__global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ cds,
+                                                   const BlockRef *__restrict__ map,
+                                                   const uint32_t *__restrict__ repp, int collect)
 {
+    SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
-    const int idx = blockIdx.x * kNT + threadIdx.x;
+    const SGPU_G int32_t *nall = g(C.nall);
+    const SGPU_G int32_t *nApart = g(C.nApart);
+    const SGPU_G double *eps = g(C.eps);
+    const SGPU_G double *F = g(C.F);
+    const SGPU_G double *lambda = g(C.lambda);
+    SGPU_G double *P = g(C.P);
+    SGPU_G double *PSum = g(C.PSum);
+    const int n_it = C.n_it;
+    const int idx = bx * kNT + threadIdx.x;
     if (idx >= m.L * m.K) {
         return;
     }
@@ -46,7 +55,7 @@ __global__ __launch_bounds__(kNT) void k_update_p(ModelArgs m, const int32_t *__
     const int na = nall[l];
     const size_t base = ((size_t)l * m.K + k) * m.A;
     const size_t stride = (size_t)m.L * m.K * m.A;
-    double g[ASTORE > 0 ? ASTORE : 1];
+    double gd[ASTORE > 0 ? ASTORE : 1];
     double sum = 0.0;
     const double Fk = F[k];
     for (int aa = 0; aa < na; aa++) {
@@ -63,7 +72,7 @@ __global__ __launch_bounds__(kNT) void k_update_p(ModelArgs m, const int32_t *__
         const double x = gamma_draw(par, m.seed, m.chain, rep, (uint32_t)l,
                                     ((uint32_t)k << 16) | (uint32_t)aa, PH_P);
         if (ASTORE > 0) {
-            g[aa] = x;
+            gd[aa] = x;
         }
         sum += x;
     }
@@ -71,7 +80,7 @@ __global__ __launch_bounds__(kNT) void k_update_p(ModelArgs m, const int32_t *__
     for (int aa = 0; aa < na; aa++) {
         double x;
         if (ASTORE > 0) {
-            x = g[aa];
+            x = gd[aa];
         } else {
             // recompute the identical draw (same counter) instead of storing it
             int n = 0;
@@ -96,8 +105,9 @@ __global__ __launch_bounds__(kNT) void k_update_p(ModelArgs m, const int32_t *__
 // UpdateQ.c:387-424 / RDirichlet ran.c:348-359), or the Metropolis proposal
 // Dirichlet(alpha) (RDirichletDisc, ran.c:1198-1210) when counts is null.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void dirichlet_q(const ModelArgs &m, const double *alpha,
-                                            const int32_t *cnt, uint32_t rep, int i,
+template <class PA>
+__device__ __forceinline__ void dirichlet_q(const ModelArgs &m, PA alpha, const int32_t *cnt,
+                                            uint32_t rep, int i,
                                             uint32_t phase, double *q)
 {
     double sum = 0.0;
@@ -113,11 +123,12 @@ __device__ __forceinline__ void dirichlet_q(const ModelArgs &m, const double *al
 
 // Store Q[i], add QSum, and accumulate the log-Q statistics the alpha update
 // reads: per-pop sum of log Q (LogProbQonepop, UpdateAlpha.c:102-111) and the
-// total over the clamped Q (LogProbQ, UpdateAlpha.c:157-180).
-template <int NT>
-__device__ __forceinline__ void finish_q(const ModelArgs &m, int i, bool active, const double *q,
-                                         double *Q, double *QSum, int collect, double *logq_part,
-                                         double *s_red)
+// total over the clamped Q (LogProbQ, UpdateAlpha.c:157-180).  Block bx
+// writes row bx of logq_part; block 0 records the row count.
+template <int NT, class PQ>
+__device__ __forceinline__ void finish_q(const ModelArgs &m, int bx, int nrows, int i, bool active,
+                                         const double *q, PQ Q, PQ QSum, int collect, PQ logq_part,
+                                         SGPU_G int32_t *logq_rows, double *s_red)
 {
     double tot = 0.0;
     for (int k = 0; k < m.K; k++) {
@@ -136,8 +147,11 @@ __device__ __forceinline__ void finish_q(const ModelArgs &m, int i, bool active,
         }
         const double s = block_sum<NT>(v, s_red);
         if (threadIdx.x == 0) {
-            logq_part[(size_t)blockIdx.x * (m.K + 1) + k] = s;
+            logq_part[(size_t)bx * (m.K + 1) + k] = s;
         }
+    }
+    if (bx == 0 && threadIdx.x == 0) {
+        logq_rows[0] = nrows;
     }
 }
 
@@ -145,18 +159,24 @@ constexpr int kGibbsInds = 64;     // individuals per UpdateQ block
 constexpr int kRowSplit = 16;      // second-level partials of the ancestry counts
 
 // First level of the ancestry-count reduction: the n_lt locus-tile partials of
-// UpdateZ ([n_lt][N*K]) are folded into kRowSplit partials; grid (flat chunks,
-// kRowSplit), each thread sums every kRowSplit-th row of one flat index with
-// four independent accumulators.
-__global__ __launch_bounds__(kNT) void k_fold_rows(const int32_t *__restrict__ part, int nrows,
-                                                   size_t n, int32_t *__restrict__ out)
+// UpdateZ ([n_lt][N*K]) are folded into kRowSplit partials; chain grid (flat
+// chunks, kRowSplit), each thread sums every kRowSplit-th row of one flat
+// index with four independent accumulators.
+__global__ __launch_bounds__(kNT) void k_fold_rows(const ChainDesc *__restrict__ cds,
+                                                   const BlockRef *__restrict__ map)
 {
-    const size_t f = (size_t)blockIdx.x * kNT + threadIdx.x;
+    SGPU_BLOCK_PROLOGUE();
+    const int by = (int)br_.by;
+    const SGPU_G int32_t *part = g(C.nQpart);
+    SGPU_G int32_t *out = g(C.nQfold);
+    const int nrows = C.n_lt;
+    const size_t n = (size_t)m.N * m.K;
+    const size_t f = (size_t)bx * kNT + threadIdx.x;
     if (f >= n) {
         return;
     }
     int32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    int t = blockIdx.y;
+    int t = by;
     for (; t + 3 * kRowSplit < nrows; t += 4 * kRowSplit) {
         a0 += part[(size_t)t * n + f];
         a1 += part[(size_t)(t + kRowSplit) * n + f];
@@ -166,19 +186,22 @@ __global__ __launch_bounds__(kNT) void k_fold_rows(const int32_t *__restrict__ p
     for (; t < nrows; t += kRowSplit) {
         a0 += part[(size_t)t * n + f];
     }
-    out[(size_t)blockIdx.y * n + f] = (a0 + a1) + (a2 + a3);
+    out[(size_t)by * n + f] = (a0 + a1) + (a2 + a3);
 }
 
 // Gibbs update of Q from the folded counts ([kRowSplit][N][K]): one thread per
 // individual, 64 per block.
-__global__ __launch_bounds__(kGibbsInds) void k_update_q_gibbs(
-    ModelArgs m, const int32_t *__restrict__ nQfold, const double *__restrict__ alpha,
-    double *__restrict__ Q, double *__restrict__ QSum, int collect,
-    double *__restrict__ logq_part, const uint32_t *__restrict__ repp)
+__global__ __launch_bounds__(kGibbsInds) void k_update_q_gibbs(const ChainDesc *__restrict__ cds,
+                                                               const BlockRef *__restrict__ map,
+                                                               const uint32_t *__restrict__ repp,
+                                                               int collect)
 {
+    SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    const SGPU_G int32_t *nQfold = g(C.nQfold);
+    const SGPU_G double *alpha = g(C.alpha);
     __shared__ double s_red[1];
-    const int i = blockIdx.x * kGibbsInds + threadIdx.x;
+    const int i = bx * kGibbsInds + threadIdx.x;
     const bool active = i < m.N;
     double q[kMaxK];
     if (active) {
@@ -193,15 +216,19 @@ __global__ __launch_bounds__(kGibbsInds) void k_update_q_gibbs(
         }
         dirichlet_q(m, alpha, cnt, rep, i, PH_Q, q);
     }
-    finish_q<kGibbsInds>(m, i, active, q, Q, QSum, collect, logq_part, s_red);
+    finish_q<kGibbsInds>(m, bx, (m.N + kGibbsInds - 1) / kGibbsInds, i, active, q, g(C.Q),
+                         g(C.QSum), collect, g(C.logq_part), g(C.logq_rows), s_red);
 }
 
-// Log-Q statistics of the current Q (after a host write of Q).
-__global__ __launch_bounds__(kNT) void k_logq_stats(ModelArgs m, const double *__restrict__ Q,
-                                                     double *__restrict__ logq_part)
+// Log-Q statistics of the current Q (after initialisation or a host write of Q).
+__global__ __launch_bounds__(kNT) void k_logq_stats(const ChainDesc *__restrict__ cds,
+                                                    const BlockRef *__restrict__ map)
 {
+    SGPU_BLOCK_PROLOGUE();
+    const SGPU_G double *Q = g(C.Q);
+    SGPU_G double *logq_part = g(C.logq_part);
     __shared__ double s_red[kNT / 64];
-    const int i = blockIdx.x * kNT + threadIdx.x;
+    const int i = bx * kNT + threadIdx.x;
     const bool active = i < m.N;
     double q[kMaxK];
     for (int k = 0; k < m.K; k++) {
@@ -215,8 +242,11 @@ __global__ __launch_bounds__(kNT) void k_logq_stats(ModelArgs m, const double *_
         const double v = active ? ((k < m.K) ? log(q[k]) : tot) : 0.0;
         const double s = block_sum<kNT>(v, s_red);
         if (threadIdx.x == 0) {
-            logq_part[(size_t)blockIdx.x * (m.K + 1) + k] = s;
+            logq_part[(size_t)bx * (m.K + 1) + k] = s;
         }
+    }
+    if (bx == 0 && threadIdx.x == 0) {
+        g(C.logq_rows)[0] = C.q_blocks;
     }
 }
 
@@ -235,18 +265,21 @@ constexpr int kMetroLoci = 256;   // loci of P staged in LDS at a time
 // the dot products (the ratio feeds an exp() compared with a uniform; it
 // needs no reference-exact rounding).
 template <int K>
-__global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(ModelArgs m,
-                                                           const uint32_t *__restrict__ geno,
-                                                           int Lpad, const double *__restrict__ Q,
-                                                           const double *__restrict__ P,
-                                                           const double *__restrict__ alpha,
-                                                           int loci_per_range,
-                                                           double *__restrict__ delta_part,
+__global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__restrict__ cds,
+                                                           const BlockRef *__restrict__ map,
                                                            const uint32_t *__restrict__ repp)
 {
+    SGPU_BLOCK_PROLOGUE();
+    const int by = (int)br_.by;
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    const SGPU_G uint32_t *geno = g(C.geno);
+    const SGPU_G double *Q = g(C.Q);
+    const SGPU_G double *P = g(C.P);
+    const SGPU_G double *alpha = g(C.alpha);
+    SGPU_G double *delta_part = g(C.delta_part);
+    const int Lpad = C.Lpad, loci_per_range = C.loci_per_range;
     __shared__ double s_p[kMetroLoci * K * 2];
-    const int i = blockIdx.x * kNT + threadIdx.x;
+    const int i = bx * kNT + threadIdx.x;
     const bool active = i < m.N;
     double tq[K], cq[K];
     {
@@ -260,9 +293,9 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(ModelArgs m,
             cq[k] = active ? Q[(size_t)i * K + k] : 1.0;
         }
     }
-    const int l0 = blockIdx.y * loci_per_range;
+    const int l0 = by * loci_per_range;
     const int l1 = min(l0 + loci_per_range, m.L);
-    const uint32_t *row = geno + (size_t)(active ? (i >> 4) : 0) * Lpad;
+    const SGPU_G uint32_t *row = geno + (size_t)(active ? (i >> 4) : 0) * Lpad;
     const int sh = 2 * (i & 15);
     double pp = 1.0, pm = 1.0;
     int ep = 0, em = 0;
@@ -274,7 +307,7 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(ModelArgs m,
         }
         __syncthreads();
         for (int lb = c0; lb < c1; lb += 4) {
-            const uint4 w4 = *(const uint4 *)(row + lb);
+            const uint4 w4 = *(const SGPU_G uint4 *)(row + lb);
             const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -308,22 +341,25 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(ModelArgs m,
         }
     }
     if (active) {
-        delta_part[(size_t)blockIdx.y * m.N + i] = (log(pp) - log(pm)) + (double)(ep - em) * kLn2;
+        delta_part[(size_t)by * m.N + i] = (log(pp) - log(pm)) + (double)(ep - em) * kLn2;
     }
 }
 
 // Byte-per-copy path (any allele count, runtime K).
-__global__ __launch_bounds__(kNT) void k_metro_ratio_byte(ModelArgs m,
-                                                           const uint32_t *__restrict__ geno,
-                                                           int Lpad, const double *__restrict__ Q,
-                                                           const double *__restrict__ P,
-                                                           const double *__restrict__ alpha,
-                                                           int loci_per_range,
-                                                           double *__restrict__ delta_part,
+__global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__restrict__ cds,
+                                                           const BlockRef *__restrict__ map,
                                                            const uint32_t *__restrict__ repp)
 {
+    SGPU_BLOCK_PROLOGUE();
+    const int by = (int)br_.by;
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
-    const int i = blockIdx.x * kNT + threadIdx.x;
+    const SGPU_G uint32_t *geno = g(C.geno);
+    const SGPU_G double *Q = g(C.Q);
+    const SGPU_G double *P = g(C.P);
+    const SGPU_G double *alpha = g(C.alpha);
+    SGPU_G double *delta_part = g(C.delta_part);
+    const int Lpad = C.Lpad, loci_per_range = C.loci_per_range;
+    const int i = bx * kNT + threadIdx.x;
     if (i >= m.N) {
         return;
     }
@@ -333,7 +369,7 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(ModelArgs m,
     for (int k = 0; k < K; k++) {
         cq[k] = Q[(size_t)i * K + k];
     }
-    const int l0 = blockIdx.y * loci_per_range;
+    const int l0 = by * loci_per_range;
     const int l1 = min(l0 + loci_per_range, m.L);
     double pp = 1.0, pm = 1.0;
     int ep = 0, em = 0;
@@ -341,7 +377,7 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(ModelArgs m,
         const uint32_t word = geno[(size_t)(i >> 1) * Lpad + l];
         const int xa = (int)((word >> (16 * (i & 1))) & 0xffu);
         const int xb = (int)((word >> (16 * (i & 1) + 8)) & 0xffu);
-        const double *Pl = P + (size_t)l * K * m.A;
+        const SGPU_G double *Pl = P + (size_t)l * K * m.A;
         if (xa != 0xff) {
             double tp = 0.0, tm = 0.0;
             for (int k = 0; k < K; k++) {
@@ -366,20 +402,22 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(ModelArgs m,
         pm = frexp(pm, &e);
         em += e;
     }
-    delta_part[(size_t)blockIdx.y * m.N + i] = (log(pp) - log(pm)) + (double)(ep - em) * kLn2;
+    delta_part[(size_t)by * m.N + i] = (log(pp) - log(pm)) + (double)(ep - em) * kLn2;
 }
 
-__global__ __launch_bounds__(kNT) void k_metro_accept(ModelArgs m, int n_ranges,
-                                                       const double *__restrict__ delta_part,
-                                                       const double *__restrict__ alpha,
-                                                       double *__restrict__ Q,
-                                                       double *__restrict__ QSum, int collect,
-                                                       double *__restrict__ logq_part,
-                                                       int32_t *__restrict__ accepted, const uint32_t *__restrict__ repp)
+__global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restrict__ cds,
+                                                       const BlockRef *__restrict__ map,
+                                                       const uint32_t *__restrict__ repp,
+                                                       int collect)
 {
+    SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    const SGPU_G double *delta_part = g(C.delta_part);
+    const SGPU_G double *alpha = g(C.alpha);
+    SGPU_G double *Q = g(C.Q);
+    const int n_ranges = C.n_ranges;
     __shared__ double s_red[kNT / 64];
-    const int i = blockIdx.x * kNT + threadIdx.x;
+    const int i = bx * kNT + threadIdx.x;
     const bool active = i < m.N;
     double q[kMaxK];
     if (active) {
@@ -397,24 +435,27 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(ModelArgs m, int n_ranges,
                 q[k] = Q[(size_t)i * m.K + k];
             }
         }
-        if (accepted) {
-            accepted[i] = acc;
-        }
     }
-    finish_q<kNT>(m, i, active, q, Q, QSum, collect, logq_part, s_red);
+    finish_q<kNT>(m, bx, C.q_blocks, i, active, q, Q, g(C.QSum), collect, g(C.logq_part),
+                  g(C.logq_rows), s_red);
 }
 
 // ---------------------------------------------------------------------------
-// UpdateAlpha (UpdateAlpha.c:189-253), one block: sums the log-Q statistics
-// and runs the pop loop in thread 0.  LogProbQ(a) = (a-1) * sum log(clamped Q)
-// + (lgamma(K a) - K lgamma(a)) * N; LogProbQonepop as in the reference.
-// As there, logprobdiff is zeroed once and only reset by the gamma prior.
+// UpdateAlpha (UpdateAlpha.c:189-253), one block per chain: sums the log-Q
+// statistics and runs the pop loop in thread 0.  LogProbQ(a) = (a-1) * sum
+// log(clamped Q) + (lgamma(K a) - K lgamma(a)) * N; LogProbQonepop as in the
+// reference.  As there, logprobdiff is zeroed once and only reset by the
+// gamma prior.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNT) void k_update_alpha(ModelArgs m, const double *__restrict__ logq_part,
-                                                       int nparts, double *__restrict__ alpha,
+__global__ __launch_bounds__(kNT) void k_update_alpha(const ChainDesc *__restrict__ cds,
+                                                       const BlockRef *__restrict__ map,
                                                        const uint32_t *__restrict__ repp)
 {
+    SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    const SGPU_G double *logq_part = g(C.logq_part);
+    SGPU_G double *alpha = g(C.alpha);
+    const int nparts = g(C.logq_rows)[0];
     __shared__ double s_red[kNT / 64];
     __shared__ double s_stat[kMaxK + 1];
     for (int k = 0; k <= m.K; k++) {
@@ -477,16 +518,19 @@ __global__ __launch_bounds__(kNT) void k_update_alpha(ModelArgs m, const double 
 // ---------------------------------------------------------------------------
 // Lambda (UpdateLambda.c:8-102).  Pass 1: per-locus log sums (loci with more
 // than one allele): sum_a log eps (F model), sum_k sum_a log P, or per pop
-// sum_a log P (pop-specific).  Pass 2 (one block): the normaliser terms are
-// grouped by allele count (ahist = (value, number of loci) pairs).
+// sum_a log P (pop-specific).  Pass 2 (one block per chain): the normaliser
+// terms are grouped by allele count (ahist = (value, number of loci) pairs).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNT) void k_lambda_partial(ModelArgs m, const int32_t *__restrict__ nall,
-                                                         const double *__restrict__ P,
-                                                         const double *__restrict__ eps,
-                                                         double *__restrict__ part)
+__global__ __launch_bounds__(kNT) void k_lambda_partial(const ChainDesc *__restrict__ cds,
+                                                         const BlockRef *__restrict__ map)
 {
+    SGPU_BLOCK_PROLOGUE();
+    const SGPU_G int32_t *nall = g(C.nall);
+    const SGPU_G double *P = g(C.P);
+    const SGPU_G double *eps = g(C.eps);
+    SGPU_G double *part = g(C.lam_part);
     __shared__ double s_red[kNT / 64];
-    const int l = blockIdx.x * kNT + threadIdx.x;
+    const int l = bx * kNT + threadIdx.x;
     const int na = (l < m.L) ? nall[l] : 0;
     const int nsums = m.popspecificlambda ? m.K : 1;
     for (int s = 0; s < nsums; s++) {
@@ -510,17 +554,21 @@ __global__ __launch_bounds__(kNT) void k_lambda_partial(ModelArgs m, const int32
         }
         const double t = block_sum<kNT>(v, s_red);
         if (threadIdx.x == 0) {
-            part[(size_t)blockIdx.x * nsums + s] = t;
+            part[(size_t)bx * nsums + s] = t;
         }
     }
 }
 
-__global__ __launch_bounds__(kNT) void k_lambda_finalize(ModelArgs m, const double *__restrict__ part,
-                                                          int nparts, const int32_t *__restrict__ ahist,
-                                                          int nhist, double *__restrict__ lambda,
+__global__ __launch_bounds__(kNT) void k_lambda_finalize(const ChainDesc *__restrict__ cds,
+                                                          const BlockRef *__restrict__ map,
                                                           const uint32_t *__restrict__ repp)
 {
+    SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    const SGPU_G double *part = g(C.lam_part);
+    const SGPU_G int32_t *ahist = g(C.ahist);
+    SGPU_G double *lambda = g(C.lambda);
+    const int nparts = C.l_blocks, nhist = C.nhist;
     __shared__ double s_red[kNT / 64];
     __shared__ double s_stat[kMaxK];
     const int nsums = m.popspecificlambda ? m.K : 1;
@@ -570,28 +618,31 @@ __global__ __launch_bounds__(kNT) void k_lambda_finalize(ModelArgs m, const doub
 }
 
 // ---------------------------------------------------------------------------
-// Epsilon (NonIndependenceUpdateEpsilon, UpdateEpsilon.c:71-145): one thread
-// per locus.  Fused: SumEpsilon += eps (output.c:684-687) when collecting.
+// Epsilon (NonIndependenceUpdateEpsilon, UpdateEpsilon.c:71-145).  One thread
+// per (locus, pop) term; a block holds whole loci (K * (256 / K) threads) so
+// the K terms of a locus meet in LDS and are summed there in the reference's
+// order.  Fused: SumEpsilon += eps (output.c:684-687) when collecting.
 // ---------------------------------------------------------------------------
-// One thread per (locus, pop) term; a block holds whole loci (K * (256 / K)
-// threads) so the K terms of a locus meet in LDS.
-__global__ __launch_bounds__(kNT) void k_update_eps(ModelArgs m, const int32_t *__restrict__ nall,
-                                                     const double *__restrict__ P,
-                                                     const double *__restrict__ F,
-                                                     const double *__restrict__ lambda,
-                                                     double *__restrict__ eps,
-                                                     double *__restrict__ epsSum, int collect,
-                                                     const uint32_t *__restrict__ repp)
+__global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict__ cds,
+                                                     const BlockRef *__restrict__ map,
+                                                     const uint32_t *__restrict__ repp, int collect)
 {
+    SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    const SGPU_G int32_t *nall = g(C.nall);
+    const SGPU_G double *P = g(C.P);
+    const SGPU_G double *F = g(C.F);
+    const SGPU_G double *lambda = g(C.lambda);
+    SGPU_G double *eps = g(C.eps);
+    SGPU_G double *epsSum = g(C.epsSum);
     __shared__ double s_term[6][kNT];   // the six terms of each (locus, pop), summed in order below
     const int K = m.K;
     const int lpb = kNT / K;                       // loci per block
     const int j = threadIdx.x / K, k = threadIdx.x % K;
-    const int l = blockIdx.x * lpb + j;
+    const int l = bx * lpb + j;
     const bool live = j < lpb && l < m.L;
     const int na = live ? nall[l] : 0;
-    double *e = eps + (size_t)(live ? l : 0) * m.A;
+    SGPU_G double *e = eps + (size_t)(live ? l : 0) * m.A;
     int a1 = 0, a2 = 0;
     double d = 0.0, e1 = 0.0, e2 = 0.0;
     bool inrange = false;
@@ -649,11 +700,12 @@ __global__ __launch_bounds__(kNT) void k_update_eps(ModelArgs m, const int32_t *
 }
 
 // ---------------------------------------------------------------------------
-// Fst (UpdateFst.c:142-204).  Every thread regenerates the proposals from the
-// stream; pass 1 sums, per locus and proposal group, the allele terms of
-// FlikeFreqs(new) - FlikeFreqs(old) (UpdateFst.c:22-51); pass 2 adds the
-// prior ratio (FPriorDiff, :10-18) and the per-locus lgamma(frac) terms and
-// decides.  ONEFST: one proposal, likelihood summed over all pops.
+// Fst (UpdateFst.c:142-204).  Pass 1 draws the proposals once per block into
+// LDS and sums, per locus and proposal group, the allele terms of
+// FlikeFreqs(new) - FlikeFreqs(old) (UpdateFst.c:22-51); pass 2 (one block
+// per chain) adds the prior ratio (FPriorDiff, :10-18) and the per-locus
+// lgamma(frac) terms and decides each group in its own thread.  ONEFST: one
+// proposal, likelihood summed over all pops.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double fst_proposal(const ModelArgs &m, double oldf, int pop1,
                                                uint32_t rep, uint32_t *uw)
@@ -665,23 +717,26 @@ __device__ __forceinline__ double fst_proposal(const ModelArgs &m, double oldf, 
     return oldf + m.fpriorsd * box_muller(w.x, w.y);
 }
 
-__global__ __launch_bounds__(kNT) void k_fst_partial(ModelArgs m, const int32_t *__restrict__ nall,
-                                                      const double *__restrict__ eps,
-                                                      const double *__restrict__ P,
-                                                      const double *__restrict__ F,
-                                                      double *__restrict__ part,
+__global__ __launch_bounds__(kNT) void k_fst_partial(const ChainDesc *__restrict__ cds,
+                                                      const BlockRef *__restrict__ map,
                                                       const uint32_t *__restrict__ repp)
 {
+    SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    const SGPU_G int32_t *nall = g(C.nall);
+    const SGPU_G double *eps = g(C.eps);
+    const SGPU_G double *P = g(C.P);
+    const SGPU_G double *F = g(C.F);
+    SGPU_G double *part = g(C.fst_part);
     __shared__ double s_red[kNT / 64];
     __shared__ double s_new[kMaxK], s_old[kMaxK];
     const int numpops1 = m.onefst ? 1 : m.K;
-    if (threadIdx.x < numpops1) {   // the block's proposals, drawn once
+    if ((int)threadIdx.x < numpops1) {   // the block's proposals, drawn once
         s_old[threadIdx.x] = F[threadIdx.x];
         s_new[threadIdx.x] = fst_proposal(m, s_old[threadIdx.x], threadIdx.x, rep, nullptr);
     }
     __syncthreads();
-    const int l = blockIdx.x * kNT + threadIdx.x;
+    const int l = bx * kNT + threadIdx.x;
     const int na = (l < m.L) ? nall[l] : 0;
     for (int p1 = 0; p1 < numpops1; p1++) {
         const double oldf = s_old[p1];
@@ -702,17 +757,20 @@ __global__ __launch_bounds__(kNT) void k_fst_partial(ModelArgs m, const int32_t 
         }
         const double t = block_sum<kNT>(v, s_red);
         if (threadIdx.x == 0) {
-            part[(size_t)blockIdx.x * numpops1 + p1] = t;
+            part[(size_t)bx * numpops1 + p1] = t;
         }
     }
 }
 
-__global__ __launch_bounds__(kNT) void k_fst_finalize(ModelArgs m, const double *__restrict__ part,
-                                                       int nparts, int nloci_nonzero,
-                                                       double *__restrict__ F,
+__global__ __launch_bounds__(kNT) void k_fst_finalize(const ChainDesc *__restrict__ cds,
+                                                       const BlockRef *__restrict__ map,
                                                        const uint32_t *__restrict__ repp)
 {
+    SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    const SGPU_G double *part = g(C.fst_part);
+    SGPU_G double *F = g(C.F);
+    const int nparts = C.l_blocks, nloci_nonzero = C.nloci_nonzero;
     __shared__ double s_red[kNT / 64];
     __shared__ double s_stat[kMaxK];
     const int numpops1 = m.onefst ? 1 : m.K;
@@ -756,18 +814,14 @@ __global__ __launch_bounds__(kNT) void k_fst_finalize(ModelArgs m, const double 
 
 // ---------------------------------------------------------------------------
 // Data-collection tail (output.c:50-101, UpdateSums :654-707): scalar sums
-// and lnP(D) from the UpdateZ partials (one block, fixed order).
+// and lnP(D) from the UpdateZ partials (one block per chain, fixed order).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNT) void k_collect(ModelArgs m, const double *__restrict__ likepart,
-                                                  int nlike, int with_like,
-                                                  const double *__restrict__ alpha,
-                                                  const double *__restrict__ lambda,
-                                                  const double *__restrict__ F,
-                                                  double *__restrict__ alphaSum,
-                                                  double *__restrict__ lambdaSum,
-                                                  double *__restrict__ FstSum,
-                                                  double *__restrict__ like3)
+__global__ __launch_bounds__(kNT) void k_collect(const ChainDesc *__restrict__ cds,
+                                                  const BlockRef *__restrict__ map, int with_like)
 {
+    SGPU_BLOCK_PROLOGUE();
+    const SGPU_G double *likepart = g(C.likepart);
+    const int nlike = C.n_it * C.n_lt;
     __shared__ double s_red[kNT / 64];
     double v = 0.0;
     if (with_like) {
@@ -780,31 +834,17 @@ __global__ __launch_bounds__(kNT) void k_collect(ModelArgs m, const double *__re
         return;
     }
     for (int k = 0; k < m.K; k++) {
-        lambdaSum[k] += lambda[k];
-        alphaSum[k] += alpha[k];
+        g(C.lambdaSum)[k] += g(C.lambda)[k];
+        g(C.alphaSum)[k] += g(C.alpha)[k];
         if (m.freqscorr) {
-            FstSum[k] += F[k];
+            g(C.FstSum)[k] += g(C.F)[k];
         }
     }
     if (with_like) {
+        SGPU_G double *like3 = g(C.like3);
         like3[0] = like;
         like3[1] += like;
         like3[2] += like * like;
-    }
-}
-
-// lnP(D) of the current state without collecting (diagnostics).
-__global__ __launch_bounds__(kNT) void k_like_only(const double *__restrict__ likepart, int nlike,
-                                                    double *__restrict__ like3)
-{
-    __shared__ double s_red[kNT / 64];
-    double v = 0.0;
-    for (int b = threadIdx.x; b < nlike; b += kNT) {
-        v += likepart[b];
-    }
-    const double like = block_sum<kNT>(v, s_red);
-    if (threadIdx.x == 0) {
-        like3[0] = like;
     }
 }
 
@@ -823,16 +863,24 @@ __global__ void k_reduce_counts(const int32_t *__restrict__ part, int nparts, si
     out[j] = s;
 }
 
-// Allele counts per locus for InitFreqPriors (Init.c:197-227), one thread per locus.
-__global__ void k_init_eps(ModelArgs m, int fmt, const uint32_t *__restrict__ geno, int Lpad,
-                           const int32_t *__restrict__ nall, double lambda0, double *__restrict__ eps)
+// Allele counts per locus for InitFreqPriors (Init.c:197-227), one thread per
+// locus; eps = lambda without the F model.
+__global__ __launch_bounds__(kNT) void k_init_eps(const ChainDesc *__restrict__ cds,
+                                                   const BlockRef *__restrict__ map,
+                                                   const double *__restrict__ lambda0s)
 {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    SGPU_BLOCK_PROLOGUE();
+    const SGPU_G uint32_t *geno = g(C.geno);
+    const SGPU_G int32_t *nall = g(C.nall);
+    SGPU_G double *eps = g(C.eps);
+    const double lambda0 = lambda0s[br_.chain];
+    const int Lpad = C.Lpad;
+    const int l = bx * kNT + threadIdx.x;
     if (l >= m.L) {
         return;
     }
     const int na = nall[l];
-    double *e = eps + (size_t)l * m.A;
+    SGPU_G double *e = eps + (size_t)l * m.A;
     for (int aa = 0; aa < m.A; aa++) {
         e[aa] = 0.0;
     }
@@ -844,7 +892,7 @@ __global__ void k_init_eps(ModelArgs m, int fmt, const uint32_t *__restrict__ ge
     }
     int total = 0;
     // counts accumulate into e[] as integers first (exact in double)
-    if (fmt == 1) {
+    if (C.fmt == 1) {
         const int G = (m.N + 15) / 16;
         for (int g = 0; g < G; g++) {
             const uint32_t word = geno[(size_t)g * Lpad + l];

@@ -85,30 +85,39 @@ struct LikeAcc {
 //                        added into lane t of a collector register by that lane
 //                        alone, and into LDS once per 16 individuals.
 // Counts per population are differences of consecutive "z > k" counts.
-// Q and P reach the kernel through restrict pointers, so Q[i,:] (wave
-// uniform) is fetched with scalar loads.
+// Q[i,:] (wave uniform) is read through the constant address space, so it is
+// fetched with scalar loads.
 // ---------------------------------------------------------------------------
-struct ZDims {
-    int N, L, A, Lpad, G, rows_per_tile;
-    uint32_t seed, chain;
-};
-
 template <int K, bool INIT, bool LIKE, bool ZOUT>
-__global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
-    const uint32_t *__restrict__ geno, const double *__restrict__ Q,
-    const double *__restrict__ P, int32_t *__restrict__ nApart, int32_t *__restrict__ nQpart,
-    double *__restrict__ likepart, int8_t *__restrict__ zout, ZDims d,
-    const uint32_t *__restrict__ repp)
+__global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__restrict__ cds,
+                                                          const BlockRef *__restrict__ map,
+                                                          const uint32_t *__restrict__ repp)
 {
+    const BlockRef br = map[blockIdx.x];
+    const ChainDesc &C = cds[br.chain];
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    const int bx = (int)br.bx, by = (int)br.by;
+    const int N = C.m.N, L = C.m.L, A = C.m.A, Lpad = C.Lpad, G = C.G;
+    const int rows_per_tile = C.rows_per_tile, n_lt = C.n_lt;
+    const uint32_t seed = C.m.seed, chain = C.m.chain;
+    const SGPU_G uint32_t *geno = g(C.geno);
+    // Q is not written during the launch: the constant address space lets the
+    // wave-uniform Q[i,:] reads go through scalar loads
+    const __attribute__((address_space(4))) double *Q =
+        (const __attribute__((address_space(4))) double *)C.Q;
+    const SGPU_G double *P = g(C.P);
+    SGPU_G int32_t *nApart = g(C.nApart);
+    SGPU_G int32_t *nQpart = g(C.nQpart);
+    SGPU_G double *likepart = g(C.likepart);
+    SGPU_G int8_t *zout = g(C.zout);
     extern __shared__ uint32_t s_cnt[];   // [tile individual][K]: valid copies, then z > k
     const int lane = threadIdx.x & 63;
-    const int l = blockIdx.x * kZLoci + threadIdx.x;   // < Lpad by construction
-    const bool lok = l < d.L;
-    const int row0 = blockIdx.y * d.rows_per_tile;
-    const int row1 = min(row0 + d.rows_per_tile, d.G);
+    const int l = bx * kZLoci + threadIdx.x;   // < Lpad by construction
+    const bool lok = l < L;
+    const int row0 = by * rows_per_tile;
+    const int row1 = min(row0 + rows_per_tile, G);
     const int ibase = row0 * 16;
-    const int ntile = min((row1 - row0) * 16, d.N - ibase);
+    const int ntile = min((row1 - row0) * 16, N - ibase);
 
     for (int t = threadIdx.x; t < ntile * K; t += kZLoci) {
         s_cnt[t] = 0;
@@ -121,8 +130,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
         P0[k] = 0.0;
         P1[k] = 0.0;
         if (!INIT && lok) {
-            P0[k] = P[((size_t)l * K + k) * d.A + 0];
-            P1[k] = P[((size_t)l * K + k) * d.A + 1];
+            P0[k] = P[((size_t)l * K + k) * A + 0];
+            P1[k] = P[((size_t)l * K + k) * A + 1];
         }
     }
     uint32_t T[K], A1[K];   // entries 0..K-2 used
@@ -134,9 +143,9 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
     uint32_t Ttot = 0, A1tot = 0;
     LikeAcc acc;
 
-    uint32_t word = geno[(size_t)row0 * d.Lpad + l];
+    uint32_t word = geno[(size_t)row0 * Lpad + l];
     for (int g = row0; g < row1; g++) {
-        const uint32_t next = (g + 1 < row1) ? geno[(size_t)(g + 1) * d.Lpad + l] : 0u;
+        const uint32_t next = (g + 1 < row1) ? geno[(size_t)(g + 1) * Lpad + l] : 0u;
         uint32_t col[K];   // collector: lane t holds the counts of individual 16g+t
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -144,15 +153,15 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
         }
         for (int jp = 0; jp < 8; jp++) {
             const int i0 = g * 16 + 2 * jp;
-            if (i0 >= d.N) {
+            if (i0 >= N) {
                 break;
             }
             const u32x4 w = philox4x32_10(rep, (uint32_t)l, (uint32_t)(i0 >> 1),
-                                          (INIT ? PH_INITZ : PH_Z) << 24, d.seed, d.chain);
+                                          (INIT ? PH_INITZ : PH_Z) << 24, seed, chain);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int i = i0 + h;
-                if (i >= d.N) {
+                if (i >= N) {
                     break;
                 }
                 const int t = 2 * jp + h;
@@ -249,8 +258,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
                     }
                 }
                 if (ZOUT && lok) {
-                    zout[((size_t)i * 2 + 0) * d.L + l] = valid ? (int8_t)z0 : (int8_t)-9;
-                    zout[((size_t)i * 2 + 1) * d.L + l] = valid ? (int8_t)z1 : (int8_t)-9;
+                    zout[((size_t)i * 2 + 0) * L + l] = valid ? (int8_t)z0 : (int8_t)-9;
+                    zout[((size_t)i * 2 + 1) * L + l] = valid ? (int8_t)z1 : (int8_t)-9;
                 }
             }
             if (LIKE) {
@@ -269,7 +278,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
     }
 
     if (lok) {
-        int32_t *dst = nApart + ((size_t)blockIdx.y * d.L + l) * K * d.A;
+        int32_t *dst = nApart + ((size_t)by * L + l) * K * A;
         uint32_t prevT = Ttot, prevA = A1tot;
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -277,8 +286,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
             const uint32_t ak = (k < K - 1) ? A1[k] : 0u;
             const uint32_t n1 = prevA - ak;
             const uint32_t nt = prevT - tk;
-            dst[k * d.A + 0] = (int32_t)(nt - n1);
-            dst[k * d.A + 1] = (int32_t)n1;
+            dst[k * A + 0] = (int32_t)(nt - n1);
+            dst[k * A + 1] = (int32_t)n1;
             prevT = tk;
             prevA = ak;
         }
@@ -286,7 +295,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
     __syncthreads();
     // convert "valid, z > 0, ..., z > K-2" into counts per population
     for (int t = threadIdx.x; t < ntile; t += kZLoci) {
-        int32_t *dst = nQpart + ((size_t)blockIdx.x * d.N + ibase + t) * K;
+        int32_t *dst = nQpart + ((size_t)bx * N + ibase + t) * K;
         uint32_t prev = s_cnt[t * K + 0];
         for (int k = 0; k < K; k++) {
             const uint32_t gk = (k < K - 1) ? s_cnt[t * K + 1 + k] : 0u;
@@ -299,7 +308,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
         const double v = lok ? acc.log_value() : 0.0;
         const double tot = block_sum<kZLoci>(v, s_red);
         if (threadIdx.x == 0) {
-            likepart[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
+            likepart[(size_t)by * n_lt + bx] = tot;
         }
     }
 }
@@ -310,15 +319,30 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(
 // lane owns its locus's partial histogram row and updates it in place.
 // ---------------------------------------------------------------------------
 template <bool INIT, bool LIKE>
-__global__ __launch_bounds__(kZLoci) void k_update_z_byte(ZArgs a, const uint32_t *__restrict__ repp)
+__global__ __launch_bounds__(kZLoci) void k_update_z_byte(const ChainDesc *__restrict__ cds,
+                                                          const BlockRef *__restrict__ map,
+                                                          const uint32_t *__restrict__ repp)
 {
+    const BlockRef br = map[blockIdx.x];
+    const ChainDesc &C = cds[br.chain];
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    const int bx = (int)br.bx, by = (int)br.by;
+    struct {
+        const uint32_t *geno;
+        const double *Q, *P;
+        int32_t *nApart, *nQpart;
+        double *likepart;
+        int8_t *zout;
+        int N, L, A, Lpad, G, rows_per_tile, n_lt;
+        uint32_t seed, chain;
+    } a = {C.geno, C.Q, C.P, C.nApart, C.nQpart, C.likepart, C.zout, C.m.N, C.m.L, C.m.A,
+           C.Lpad, C.G, C.rows_per_tile, C.n_lt, C.m.seed, C.m.chain};
     extern __shared__ uint32_t s_cnt[];
     const int lane = threadIdx.x & 63;
-    const int K = a.K;
-    const int l = blockIdx.x * kZLoci + threadIdx.x;
+    const int K = C.m.K;
+    const int l = bx * kZLoci + threadIdx.x;
     const bool lok = l < a.L;
-    const int row0 = blockIdx.y * a.rows_per_tile;
+    const int row0 = by * a.rows_per_tile;
     const int row1 = min(row0 + a.rows_per_tile, a.G);
     const int ibase = row0 * 2;
     const int ntile = min((row1 - row0) * 2, a.N - ibase);
@@ -326,7 +350,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_byte(ZArgs a, const uint32_
     for (int t = threadIdx.x; t < ntile * K; t += kZLoci) {
         s_cnt[t] = 0;
     }
-    int32_t *hist = a.nApart + ((size_t)blockIdx.y * a.L + (lok ? l : 0)) * K * a.A;
+    int32_t *hist = a.nApart + ((size_t)by * a.L + (lok ? l : 0)) * K * a.A;
     if (lok) {
         for (int t = 0; t < K * a.A; t++) {
             hist[t] = 0;
@@ -411,14 +435,14 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_byte(ZArgs a, const uint32_
     }
     __syncthreads();
     for (int t = threadIdx.x; t < ntile * K; t += kZLoci) {
-        a.nQpart[((size_t)blockIdx.x * a.N + ibase) * K + t] = (int32_t)s_cnt[t];
+        a.nQpart[((size_t)bx * a.N + ibase) * K + t] = (int32_t)s_cnt[t];
     }
     if (LIKE) {
         __shared__ double s_red[kZLoci / 64];
         const double v = lok ? acc.log_value() : 0.0;
         const double tot = block_sum<kZLoci>(v, s_red);
         if (threadIdx.x == 0) {
-            a.likepart[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
+            a.likepart[(size_t)by * a.n_lt + bx] = tot;
         }
     }
 }

@@ -1,6 +1,13 @@
 // C-ABI of the MI355X admixture sweep (include/sgpu.h): context, device
 // state, and the per-phase entry points that replace the reference's *CL
 // wrappers and its OpenCL dispatch layer (Kernels.c).
+//
+// A context holds a batch of chains (one chain for sgpu_create).  Each phase
+// is one launch per kernel over every chain of the batch: block b of a launch
+// reads its chain and its block coordinates from a block map built once at
+// creation, and the chain's shapes and state addresses from a device array of
+// chain descriptors.  UpdateZ and the Metropolis ratio are compiled per K, so
+// they launch once per distinct (genotype format, K) among the chains.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,44 +56,68 @@ int fail(int code, const char *fmt, ...)
 
 size_t cdiv(size_t a, size_t b) { return (a + b - 1) / b; }
 
+// Default block targets of the tiled phases (about 8 blocks per CU on 256 CUs).
+constexpr int kZBlocksDefault = 2048;
+constexpr int kMetroBlocksDefault = 2048;
+// LDS per UpdateZ block for the per-individual counts (keeps 3 blocks per CU).
+constexpr size_t kZLdsMax = 48 * 1024;
+
+// One launch over a set of chains: the block map and its LDS need.
+struct Plan {
+    BlockRef *map = nullptr;
+    int nblocks = 0;
+    size_t lds = 0;
+};
+
+// A launch of a K-templated kernel: the chains of one (format, K).
+struct KGroup {
+    int fmt = 0, K = 0;
+    Plan plan;
+};
+
+struct Dataset {
+    int fmt = 0, N = 0, L = 0, A = 0, Lpad = 0, G = 0;
+    uint32_t *geno = nullptr;
+    int32_t *nall = nullptr, *ahist = nullptr;
+    int nhist = 0, nloci_nonzero = 0;
+};
+
+struct HostChain {
+    sgpu_model m{};
+    int dataset = 0;
+    size_t z_lds = 0;
+    ChainDesc d{};   // host copy of the device descriptor
+    int32_t *nA = nullptr, *nQ = nullptr;   // reduced counts for state reads
+};
+
 }  // namespace
 
 struct sgpu_ctx {
-    sgpu_model m{};
-    ModelArgs ma{};
-    int fmt = 0;
     int device = 0;
     hipStream_t stream = nullptr;
-    int Lpad = 0, G = 0, inds_per_row = 0;
-    // UpdateZ tiling
-    int n_lt = 0, n_it = 0, rows_per_tile = 0;
-    size_t z_lds = 0;
-    // Metropolis tiling
-    int n_ranges = 0, loci_per_range = 0;
-    // partial-buffer block counts
-    int q_blocks = 0, l_blocks = 0;
-    int logq_rows = 0;   // rows of logq_part written by the last Q update
-    // device state
-    uint32_t *geno = nullptr;
-    int32_t *nall = nullptr, *ahist = nullptr;
-    double *Q = nullptr, *P = nullptr, *eps = nullptr, *F = nullptr, *alpha = nullptr,
-           *lambda = nullptr;
-    int32_t *nApart = nullptr, *nQpart = nullptr, *nA = nullptr, *nQ = nullptr, *nQfold = nullptr;
-    double *likepart = nullptr, *logq_part = nullptr, *delta_part = nullptr,
-           *lam_part = nullptr, *fst_part = nullptr;
-    double *QSum = nullptr, *PSum = nullptr, *epsSum = nullptr, *FstSum = nullptr,
-           *alphaSum = nullptr, *lambdaSum = nullptr, *like3 = nullptr;
-    int8_t *zdbg = nullptr;
-    uint32_t *repd = nullptr;     // device copy of the sweep index read by every kernel
-    uint32_t rep_shadow = 0;      // host view of *repd
+    std::vector<Dataset> data;
+    std::vector<HostChain> chains;
+    ChainDesc *cds = nullptr;       // device descriptors [nchains]
+    double *lambda0 = nullptr;      // device [nchains], for InitFreqPriors
+    // batch-wide cadence (equal for every chain)
+    int32_t burnin = 0, metrofreq = 0, computeprob = 0;
+    bool any_alpha = false, any_lambda = false, any_freqscorr = false;
+    // launch plans
+    std::vector<KGroup> z_groups, metro_groups;
+    Plan p_update_p8, p_update_p0, p_fold, p_gibbs, p_logq, p_accept, p_alpha, p_lam_part,
+        p_lam_fin, p_eps, p_fst_part, p_fst_fin, p_collect, p_init_eps;
+    std::vector<Plan> p_logq_chain;   // per chain, after a host write of Q
+    // device sweep index read by every kernel
+    uint32_t *repd = nullptr;
+    uint32_t rep_shadow = 0;          // host view of *repd
     bool rep_known = false;
     bool capturing = false;
     bool use_graphs = true;
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};   // [metro | 2*collect]
-    int nhist = 0, nloci_nonzero = 0;
     int64_t nsum = 0;
     bool initialized = false;
     bool like_ready = false;
+    int8_t *zdbg = nullptr;
     std::vector<void *> allocs;
     // phase timing (sgpu_set_timing)
     struct Pending {
@@ -128,6 +159,8 @@ ModelArgs model_args(const sgpu_model &m)
     a.popalphas = m.popalphas;
     a.unifprioralpha = m.unifprioralpha;
     a.popspecificlambda = m.popspecificlambda;
+    a.inferalpha = m.inferalpha;
+    a.inferlambda = m.inferlambda;
     a.alphapropsd = m.alphapropsd;
     a.alphamax = m.alphamax;
     a.alphapriora = m.alphapriora;
@@ -139,31 +172,40 @@ ModelArgs model_args(const sgpu_model &m)
     return a;
 }
 
-ZArgs z_args(sgpu_ctx *c, uint32_t rep)
+// Block map over the chains selected by sel; dims(chain) gives the chain's
+// own grid (nx, ny).  Blocks are listed chain by chain, x fastest.
+template <class Sel, class Dims>
+int build_plan(sgpu_ctx *c, Plan &p, Sel sel, Dims dims)
 {
-    ZArgs a{};
-    a.geno = c->geno;
-    a.Q = c->Q;
-    a.P = c->P;
-    a.nApart = c->nApart;
-    a.nQpart = c->nQpart;
-    a.likepart = c->likepart;
-    a.zout = nullptr;
-    a.N = c->m.N;
-    a.L = c->m.L;
-    a.K = c->m.K;
-    a.A = c->m.A;
-    a.Lpad = c->Lpad;
-    a.G = c->G;
-    a.rows_per_tile = c->rows_per_tile;
-    a.inds_per_row = c->inds_per_row;
-    a.seed = c->m.seed;
-    a.chain = c->m.chain;
-    return a;
+    std::vector<BlockRef> v;
+    for (size_t ci = 0; ci < c->chains.size(); ci++) {
+        if (!sel(c->chains[ci])) {
+            continue;
+        }
+        const std::pair<int, int> g = dims(c->chains[ci]);
+        for (int by = 0; by < g.second; by++) {
+            for (int bx = 0; bx < g.first; bx++) {
+                v.push_back(BlockRef{(uint32_t)ci, (uint32_t)bx, (uint32_t)by, 0u});
+            }
+        }
+    }
+    p.nblocks = (int)v.size();
+    if (v.empty()) {
+        return SGPU_OK;
+    }
+    int rc = dalloc(c, &p.map, v.size());
+    if (rc) {
+        return rc;
+    }
+    HIP_OK(hipMemcpyAsync(p.map, v.data(), v.size() * sizeof(BlockRef), hipMemcpyHostToDevice,
+                          c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));   // v is freed on return
+    return SGPU_OK;
 }
 
-typedef void (*zkern_t)(const uint32_t *, const double *, const double *, int32_t *, int32_t *,
-                        double *, int8_t *, ZDims, const uint32_t *);
+auto all_chains = [](const HostChain &) { return true; };
+
+typedef void (*zkern_t)(const ChainDesc *, const BlockRef *, const uint32_t *);
 
 template <int K>
 zkern_t pick_snp2(int mode)
@@ -177,8 +219,16 @@ zkern_t pick_snp2(int mode)
     }
 }
 
-zkern_t snp2_kernel(int K, int mode)
+// mode 0 = initial Z, 1 = update, 2 = update + lnP terms (+2 with a debug copy of Z)
+zkern_t z_kernel(int fmt, int K, int mode)
 {
+    if (fmt == SGPU_GENO_BYTE) {
+        switch (mode) {
+        case 0: return k_update_z_byte<true, false>;
+        case 1: case 3: return k_update_z_byte<false, false>;
+        default: return k_update_z_byte<false, true>;
+        }
+    }
     switch (K) {
     case 1: return pick_snp2<1>(mode);
     case 2: return pick_snp2<2>(mode);
@@ -200,11 +250,13 @@ zkern_t snp2_kernel(int K, int mode)
     }
 }
 
-typedef void (*mkern_t)(ModelArgs, const uint32_t *, int, const double *, const double *,
-                        const double *, int, double *, const uint32_t *);
+typedef void (*mkern_t)(const ChainDesc *, const BlockRef *, const uint32_t *);
 
-mkern_t metro_snp2_kernel(int K)
+mkern_t metro_kernel(int fmt, int K)
 {
+    if (fmt == SGPU_GENO_BYTE) {
+        return k_metro_ratio_byte;
+    }
     switch (K) {
     case 1: return k_metro_ratio_snp2<1>;
     case 2: return k_metro_ratio_snp2<2>;
@@ -226,47 +278,37 @@ mkern_t metro_snp2_kernel(int K)
     }
 }
 
-// mode 0 = initial Z, 1 = update, 2 = update + lnP terms (+2 with a debug copy of Z)
-int launch_z(sgpu_ctx *c, uint32_t rep, int mode, int8_t *zdev)
+int launch_z(sgpu_ctx *c, int mode)
 {
-    ZArgs a = z_args(c, rep);
-    a.zout = zdev;
-    dim3 grid(c->n_lt, c->n_it), block(kZLoci);
-    if (c->fmt == SGPU_GENO_SNP2) {
-        zkern_t k = snp2_kernel(c->m.K, (zdev && mode > 0) ? mode + 2 : mode);
+    for (const KGroup &g : c->z_groups) {
+        zkern_t k = z_kernel(g.fmt, g.K, mode);
         if (!k) {
-            return fail(SGPU_ERR_UNSUPPORTED, "no SNP2 UpdateZ kernel for K=%d", c->m.K);
+            return fail(SGPU_ERR_UNSUPPORTED, "no UpdateZ kernel for K=%d", g.K);
         }
-        ZDims zd{};
-        zd.N = c->m.N;
-        zd.L = c->m.L;
-        zd.A = c->m.A;
-        zd.Lpad = c->Lpad;
-        zd.G = c->G;
-        zd.rows_per_tile = c->rows_per_tile;
-        zd.seed = c->m.seed;
-        zd.chain = c->m.chain;
-            hipLaunchKernelGGL(k, grid, block, c->z_lds, c->stream, c->geno, c->Q, c->P, c->nApart,
-                           c->nQpart, c->likepart, zdev, zd, (const uint32_t *)c->repd);
-    } else {
-        if (mode == 0) {
-            hipLaunchKernelGGL((k_update_z_byte<true, false>), grid, block, c->z_lds, c->stream, a,
-                               (const uint32_t *)c->repd);
-        } else if (mode == 1) {
-            hipLaunchKernelGGL((k_update_z_byte<false, false>), grid, block, c->z_lds, c->stream, a,
-                               (const uint32_t *)c->repd);
-        } else {
-            hipLaunchKernelGGL((k_update_z_byte<false, true>), grid, block, c->z_lds, c->stream, a,
-                               (const uint32_t *)c->repd);
-        }
+        hipLaunchKernelGGL(k, dim3(g.plan.nblocks), dim3(kZLoci), g.plan.lds, c->stream, c->cds,
+                           g.plan.map, (const uint32_t *)c->repd);
+        LAUNCH_OK();
     }
+    return SGPU_OK;
+}
+
+// Launch a batched kernel over a plan (nothing when the plan is empty).
+template <typename... Args>
+int launch(sgpu_ctx *c, void (*k)(const ChainDesc *, const BlockRef *, Args...), const Plan &p,
+           int nthreads, Args... args)
+{
+    if (p.nblocks == 0) {
+        return SGPU_OK;
+    }
+    hipLaunchKernelGGL(k, dim3(p.nblocks), dim3(nthreads), p.lds, c->stream, c->cds, p.map,
+                       args...);
     LAUNCH_OK();
     return SGPU_OK;
 }
 
-size_t state_elems(const sgpu_ctx *c, int which, size_t *elem)
+size_t state_elems(const sgpu_model &m, int which, size_t *elem)
 {
-    const size_t N = c->m.N, L = c->m.L, K = c->m.K, A = c->m.A;
+    const size_t N = m.N, L = m.L, K = m.K, A = m.A;
     *elem = sizeof(double);
     switch (which) {
     case SGPU_Q: case SGPU_QSUM: return N * K;
@@ -282,33 +324,24 @@ size_t state_elems(const sgpu_ctx *c, int which, size_t *elem)
     }
 }
 
-double *state_ptr(sgpu_ctx *c, int which)
+double *state_ptr(const ChainDesc &d, int which)
 {
     switch (which) {
-    case SGPU_Q: return c->Q;
-    case SGPU_P: return c->P;
-    case SGPU_EPS: return c->eps;
-    case SGPU_FST: return c->F;
-    case SGPU_ALPHA: return c->alpha;
-    case SGPU_LAMBDA: return c->lambda;
-    case SGPU_QSUM: return c->QSum;
-    case SGPU_PSUM: return c->PSum;
-    case SGPU_FSTSUM: return c->FstSum;
-    case SGPU_EPSSUM: return c->epsSum;
-    case SGPU_ALPHASUM: return c->alphaSum;
-    case SGPU_LAMBDASUM: return c->lambdaSum;
-    case SGPU_LIKE: return c->like3;
+    case SGPU_Q: return d.Q;
+    case SGPU_P: return d.P;
+    case SGPU_EPS: return d.eps;
+    case SGPU_FST: return d.F;
+    case SGPU_ALPHA: return d.alpha;
+    case SGPU_LAMBDA: return d.lambda;
+    case SGPU_QSUM: return d.QSum;
+    case SGPU_PSUM: return d.PSum;
+    case SGPU_FSTSUM: return d.FstSum;
+    case SGPU_EPSSUM: return d.epsSum;
+    case SGPU_ALPHASUM: return d.alphaSum;
+    case SGPU_LAMBDASUM: return d.lambdaSum;
+    case SGPU_LIKE: return d.like3;
     default: return nullptr;
     }
-}
-
-int launch_logq(sgpu_ctx *c)
-{
-    hipLaunchKernelGGL(k_logq_stats, dim3(c->q_blocks), dim3(kNT), 0, c->stream, c->ma, c->Q,
-                       c->logq_part);
-    LAUNCH_OK();
-    c->logq_rows = c->q_blocks;
-    return SGPU_OK;
 }
 
 hipEvent_t take_event(sgpu_ctx *c)
@@ -331,11 +364,6 @@ int drain_timing(sgpu_ctx *c)
         return SGPU_OK;
     }
     HIP_OK(hipStreamSynchronize(c->stream));
-    for (int t = 0; t < 4; t++) {
-        if (c->graph[t]) {
-            (void)hipGraphExecDestroy(c->graph[t]);
-        }
-    }
     for (auto &p : c->pending) {
         float ms = 0.f;
         HIP_OK(hipEventElapsedTime(&ms, p.a, p.b));
@@ -355,7 +383,7 @@ struct PhaseTimer {
     hipEvent_t a = nullptr;
     PhaseTimer(sgpu_ctx *ctx, int ph) : c(ctx), phase(ph)
     {
-        if (c && c->timing && (a = take_event(c))) {
+        if (c && c->timing && !c->capturing && (a = take_event(c))) {
             (void)hipEventRecord(a, c->stream);
         }
     }
@@ -405,149 +433,93 @@ int check_ready(sgpu_ctx *c)
     return SGPU_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char *sgpu_last_error(void) { return g_err; }
-
-const char *sgpu_version(void)
+int check_model(const sgpu_model *m, int ci)
 {
-    return "sgpu 0.1 gfx950: fused UpdateZ (SNP2 and byte formats, K<=16), FP64 draws";
-}
-
-int sgpu_create(const sgpu_model *m, const sgpu_data *d, sgpu_ctx **out)
-{
-    if (!m || !d || !out || !d->geno || !d->nall) {
-        return fail(SGPU_ERR_ARG, "null argument");
-    }
-    *out = nullptr;
     if (m->N < 1 || m->L < 1 || m->K < 1 || m->A < 1) {
-        return fail(SGPU_ERR_ARG, "bad shape N=%d L=%d K=%d A=%d", m->N, m->L, m->K, m->A);
+        return fail(SGPU_ERR_ARG, "chain %d: bad shape N=%d L=%d K=%d A=%d", ci, m->N, m->L, m->K,
+                    m->A);
     }
     if (m->K > kMaxK) {
-        return fail(SGPU_ERR_UNSUPPORTED, "K=%d exceeds the supported maximum %d", m->K, kMaxK);
+        return fail(SGPU_ERR_UNSUPPORTED, "chain %d: K=%d exceeds the supported maximum %d", ci,
+                    m->K, kMaxK);
     }
     if (m->A > kMaxAlleles) {
-        return fail(SGPU_ERR_UNSUPPORTED, "MAXALLELES=%d exceeds %d", m->A, kMaxAlleles);
+        return fail(SGPU_ERR_UNSUPPORTED, "chain %d: MAXALLELES=%d exceeds %d", ci, m->A,
+                    kMaxAlleles);
     }
-    if (d->format == SGPU_GENO_SNP2 && m->A != 2) {
-        return fail(SGPU_ERR_ARG, "SNP2 genotypes need MAXALLELES == 2 (got %d)", m->A);
+    if (m->freqscorr && (m->fpriormean <= 0.0 || m->fpriorsd <= 0.0)) {
+        return fail(SGPU_ERR_ARG, "chain %d: FPRIORMEAN and FPRIORSD must be positive", ci);
+    }
+    return SGPU_OK;
+}
+
+// Every allele index must lie below NumAlleles of its locus (the kernels index
+// P and the count rows with it).
+int check_data(const sgpu_data *d, int N, int L, int A, int di)
+{
+    if (!d->geno || !d->nall) {
+        return fail(SGPU_ERR_ARG, "data set %d: null genotype or NumAlleles pointer", di);
     }
     if (d->format != SGPU_GENO_SNP2 && d->format != SGPU_GENO_BYTE) {
-        return fail(SGPU_ERR_ARG, "unknown genotype format %d", d->format);
+        return fail(SGPU_ERR_ARG, "data set %d: unknown genotype format %d", di, d->format);
     }
-    const size_t need = sgpu_packed_words(d->format, m->N, m->L);
+    if (d->format == SGPU_GENO_SNP2 && A != 2) {
+        return fail(SGPU_ERR_ARG, "data set %d: SNP2 genotypes need MAXALLELES == 2 (got %d)", di,
+                    A);
+    }
+    const size_t need = sgpu_packed_words(d->format, N, L);
     if (d->geno_words != need) {
-        return fail(SGPU_ERR_ARG, "genotype buffer has %zu words, format needs %zu",
-                    d->geno_words, need);
+        return fail(SGPU_ERR_ARG, "data set %d: genotype buffer has %zu words, format needs %zu",
+                    di, d->geno_words, need);
     }
-    for (int l = 0; l < m->L; l++) {
-        if (d->nall[l] < 0 || d->nall[l] > m->A) {
-            return fail(SGPU_ERR_ARG, "NumAlleles[%d]=%d outside [0, %d]", l, d->nall[l], m->A);
+    for (int l = 0; l < L; l++) {
+        if (d->nall[l] < 0 || d->nall[l] > A) {
+            return fail(SGPU_ERR_ARG, "data set %d: NumAlleles[%d]=%d outside [0, %d]", di, l,
+                        d->nall[l], A);
         }
     }
-    // every allele index must lie below NumAlleles of its locus (the kernels index
-    // P and the count rows with it)
-    {
-        const size_t lp = need / (size_t)cdiv(m->N, d->format == SGPU_GENO_SNP2 ? 16 : 2);
-        for (size_t row = 0; row * lp < need; row++) {
-            const uint32_t *w = d->geno + row * lp;
-            for (int l = 0; l < m->L; l++) {
-                const uint32_t x = w[l];
-                if (d->format == SGPU_GENO_SNP2) {
-                    for (int j = 0; j < 16; j++) {
-                        const uint32_t code = (x >> (2 * j)) & 3u;
-                        if (code != 3u && (int)(code != 0u) >= d->nall[l]) {
-                            return fail(SGPU_ERR_ARG, "allele 1 at locus %d exceeds NumAlleles=%d", l,
-                                        d->nall[l]);
-                        }
-                        if (code == 0u && d->nall[l] < 1) {
-                            return fail(SGPU_ERR_ARG, "allele 0 at locus %d with NumAlleles=0", l);
-                        }
+    const size_t lp = need / cdiv(N, d->format == SGPU_GENO_SNP2 ? 16 : 2);
+    for (size_t row = 0; row * lp < need; row++) {
+        const uint32_t *w = d->geno + row * lp;
+        for (int l = 0; l < L; l++) {
+            const uint32_t x = w[l];
+            if (d->format == SGPU_GENO_SNP2) {
+                for (int j = 0; j < 16; j++) {
+                    const uint32_t code = (x >> (2 * j)) & 3u;
+                    if (code != 3u && (int)(code != 0u) >= d->nall[l]) {
+                        return fail(SGPU_ERR_ARG, "allele 1 at locus %d exceeds NumAlleles=%d", l,
+                                    d->nall[l]);
                     }
-                } else {
-                    for (int b = 0; b < 4; b++) {
-                        const uint32_t a = (x >> (8 * b)) & 0xffu;
-                        if (a != 0xffu && (int)a >= d->nall[l]) {
-                            return fail(SGPU_ERR_ARG, "allele %u at locus %d exceeds NumAlleles=%d",
-                                        a, l, d->nall[l]);
-                        }
+                    if (code == 0u && d->nall[l] < 1) {
+                        return fail(SGPU_ERR_ARG, "allele 0 at locus %d with NumAlleles=0", l);
+                    }
+                }
+            } else {
+                for (int b = 0; b < 4; b++) {
+                    const uint32_t a = (x >> (8 * b)) & 0xffu;
+                    if (a != 0xffu && (int)a >= d->nall[l]) {
+                        return fail(SGPU_ERR_ARG, "allele %u at locus %d exceeds NumAlleles=%d", a,
+                                    l, d->nall[l]);
                     }
                 }
             }
         }
     }
-    if (m->freqscorr && (m->fpriormean <= 0.0 || m->fpriorsd <= 0.0)) {
-        return fail(SGPU_ERR_ARG, "FPRIORMEAN and FPRIORSD must be positive");
-    }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        return fail(SGPU_ERR_DEVICE, "no HIP device available");
-    }
+    return SGPU_OK;
+}
 
-    sgpu_ctx *c = new sgpu_ctx();
-    c->m = *m;
-    c->ma = model_args(*m);
-    c->fmt = d->format;
-    if (hipGetDevice(&c->device) != hipSuccess) {
-        delete c;
-        return fail(SGPU_ERR_DEVICE, "hipGetDevice failed");
-    }
-    int rc = SGPU_OK;
-    auto cleanup = [&](int code) {
-        sgpu_destroy(c);
-        return code;
-    };
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return fail(SGPU_ERR_DEVICE, "hipStreamCreate failed");
-    }
-
-    const int N = m->N, L = m->L, K = m->K, A = m->A;
-    c->inds_per_row = d->format == SGPU_GENO_SNP2 ? 16 : 2;
-    c->G = (int)cdiv(N, c->inds_per_row);
-    c->n_lt = (int)cdiv(L, kZLoci);
-    c->Lpad = c->n_lt * kZLoci;
-    // about 2048 UpdateZ blocks (8 per CU) with per-tile LDS counts <= 48 KiB
-    int n_it = (int)std::min<size_t>(std::max<size_t>(cdiv(2048, c->n_lt), 1), c->G);
-    int rows = (int)cdiv(c->G, n_it);
-    while ((size_t)rows * c->inds_per_row * K * 4 > 48 * 1024 && rows > 1) {
-        rows = (rows + 1) / 2;
-    }
-    c->rows_per_tile = rows;
-    c->n_it = (int)cdiv(c->G, rows);
-    c->z_lds = (size_t)rows * c->inds_per_row * K * sizeof(uint32_t);
-    c->q_blocks = (int)cdiv(N, kNT);
-    c->l_blocks = (int)cdiv(L, kNT);
-    c->n_ranges = (int)std::max<size_t>(1, std::min<size_t>(cdiv(2048, c->q_blocks), L));
-    c->loci_per_range = (int)(cdiv(cdiv(L, c->n_ranges), 4) * 4);   // multiple of 4 loci
-    c->n_ranges = (int)cdiv(L, c->loci_per_range);
-
-    if ((rc = dalloc(c, &c->geno, need)) || (rc = dalloc(c, &c->nall, L)) ||
-        (rc = dalloc(c, &c->ahist, 2 * (A + 1))) || (rc = dalloc(c, &c->Q, (size_t)N * K)) ||
-        (rc = dalloc(c, &c->P, (size_t)L * K * A)) || (rc = dalloc(c, &c->eps, (size_t)L * A)) ||
-        (rc = dalloc(c, &c->F, K)) || (rc = dalloc(c, &c->alpha, K)) ||
-        (rc = dalloc(c, &c->lambda, K)) ||
-        (rc = dalloc(c, &c->nApart, (size_t)c->n_it * L * K * A)) ||
-        (rc = dalloc(c, &c->nQpart, (size_t)c->n_lt * N * K)) ||
-        (rc = dalloc(c, &c->nA, (size_t)L * K * A)) || (rc = dalloc(c, &c->nQ, (size_t)N * K)) ||
-        (rc = dalloc(c, &c->nQfold, (size_t)kRowSplit * N * K)) ||
-        (rc = dalloc(c, &c->likepart, (size_t)c->n_it * c->n_lt)) ||
-        (rc = dalloc(c, &c->logq_part, cdiv(N, kGibbsInds) * (K + 1))) ||
-        (rc = dalloc(c, &c->delta_part, (size_t)c->n_ranges * N)) ||
-        (rc = dalloc(c, &c->lam_part, (size_t)c->l_blocks * K)) ||
-        (rc = dalloc(c, &c->fst_part, (size_t)c->l_blocks * K)) ||
-        (rc = dalloc(c, &c->QSum, (size_t)N * K)) || (rc = dalloc(c, &c->PSum, (size_t)L * K * A)) ||
-        (rc = dalloc(c, &c->epsSum, (size_t)L * A)) || (rc = dalloc(c, &c->FstSum, K)) ||
-        (rc = dalloc(c, &c->alphaSum, K)) || (rc = dalloc(c, &c->lambdaSum, K)) ||
-        (rc = dalloc(c, &c->like3, 3)) || (rc = dalloc(c, &c->repd, 2))) {
-        return cleanup(rc);
+int upload_dataset(sgpu_ctx *c, Dataset &ds, const sgpu_data *d)
+{
+    const size_t need = sgpu_packed_words(d->format, ds.N, ds.L);
+    int rc;
+    if ((rc = dalloc(c, &ds.geno, need)) || (rc = dalloc(c, &ds.nall, ds.L)) ||
+        (rc = dalloc(c, &ds.ahist, 2 * (ds.A + 1)))) {
+        return rc;
     }
     // allele-count histogram over loci with more than one allele (lambda update)
     std::map<int, int> hist;
     int nzero = 0;
-    for (int l = 0; l < L; l++) {
+    for (int l = 0; l < ds.L; l++) {
         if (d->nall[l] > 1) {
             hist[d->nall[l]]++;
         }
@@ -560,18 +532,304 @@ int sgpu_create(const sgpu_model *m, const sgpu_data *d, sgpu_ctx **out)
         hv.push_back(kv.first);
         hv.push_back(kv.second);
     }
-    c->nhist = (int)hist.size();
-    c->nloci_nonzero = L - nzero;
-    hipError_t e = hipMemcpyAsync(c->geno, d->geno, need * sizeof(uint32_t), hipMemcpyHostToDevice,
-                                  c->stream);
-    if (e == hipSuccess) {
-        e = hipMemcpyAsync(c->nall, d->nall, (size_t)L * sizeof(int32_t), hipMemcpyHostToDevice,
-                           c->stream);
+    ds.nhist = (int)hist.size();
+    ds.nloci_nonzero = ds.L - nzero;
+    HIP_OK(hipMemcpyAsync(ds.geno, d->geno, need * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          c->stream));
+    HIP_OK(hipMemcpyAsync(ds.nall, d->nall, (size_t)ds.L * sizeof(int32_t), hipMemcpyHostToDevice,
+                          c->stream));
+    if (!hv.empty()) {
+        HIP_OK(hipMemcpyAsync(ds.ahist, hv.data(), hv.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, c->stream));
     }
-    if (e == hipSuccess && !hv.empty()) {
-        e = hipMemcpyAsync(c->ahist, hv.data(), hv.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                           c->stream);
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return SGPU_OK;
+}
+
+// Tiles of the chain's UpdateZ and Metropolis launches for the batch's block
+// targets, then its device state.
+int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, int total_lt, int total_qb,
+                int z_target, int metro_target)
+{
+    const sgpu_model &m = h.m;
+    ChainDesc &d = h.d;
+    const int N = m.N, L = m.L, K = m.K, A = m.A;
+    d.m = model_args(m);
+    d.fmt = ds.fmt;
+    d.Lpad = ds.Lpad;
+    d.G = ds.G;
+    d.geno = ds.geno;
+    d.nall = ds.nall;
+    d.ahist = ds.ahist;
+    d.nhist = ds.nhist;
+    d.nloci_nonzero = ds.nloci_nonzero;
+    const int inds_per_row = ds.fmt == SGPU_GENO_SNP2 ? 16 : 2;
+    d.n_lt = (int)cdiv(L, kZLoci);
+    int n_it = (int)std::min<size_t>(std::max<size_t>(cdiv(z_target, total_lt), 1), d.G);
+    int rows = (int)cdiv(d.G, n_it);
+    while ((size_t)rows * inds_per_row * K * 4 > kZLdsMax && rows > 1) {
+        rows = (rows + 1) / 2;
     }
+    d.rows_per_tile = rows;
+    d.n_it = (int)cdiv(d.G, rows);
+    h.z_lds = (size_t)rows * inds_per_row * K * sizeof(uint32_t);
+    d.q_blocks = (int)cdiv(N, kNT);
+    d.l_blocks = (int)cdiv(L, kNT);
+    int n_ranges = (int)std::max<size_t>(1, std::min<size_t>(cdiv(metro_target, total_qb), L));
+    d.loci_per_range = (int)(cdiv(cdiv(L, n_ranges), 4) * 4);   // multiple of 4 loci
+    d.n_ranges = (int)cdiv(L, d.loci_per_range);
+    d.zout = nullptr;
+
+    int rc;
+    if ((rc = dalloc(c, &d.Q, (size_t)N * K)) || (rc = dalloc(c, &d.P, (size_t)L * K * A)) ||
+        (rc = dalloc(c, &d.eps, (size_t)L * A)) || (rc = dalloc(c, &d.F, K)) ||
+        (rc = dalloc(c, &d.alpha, K)) || (rc = dalloc(c, &d.lambda, K)) ||
+        (rc = dalloc(c, &d.nApart, (size_t)d.n_it * L * K * A)) ||
+        (rc = dalloc(c, &d.nQpart, (size_t)d.n_lt * N * K)) ||
+        (rc = dalloc(c, &h.nA, (size_t)L * K * A)) || (rc = dalloc(c, &h.nQ, (size_t)N * K)) ||
+        (rc = dalloc(c, &d.nQfold, (size_t)kRowSplit * N * K)) ||
+        (rc = dalloc(c, &d.logq_rows, 1)) ||
+        (rc = dalloc(c, &d.likepart, (size_t)d.n_it * d.n_lt)) ||
+        (rc = dalloc(c, &d.logq_part, cdiv(N, kGibbsInds) * (K + 1))) ||
+        (rc = dalloc(c, &d.delta_part, (size_t)d.n_ranges * N)) ||
+        (rc = dalloc(c, &d.lam_part, (size_t)d.l_blocks * K)) ||
+        (rc = dalloc(c, &d.fst_part, (size_t)d.l_blocks * K)) ||
+        (rc = dalloc(c, &d.QSum, (size_t)N * K)) || (rc = dalloc(c, &d.PSum, (size_t)L * K * A)) ||
+        (rc = dalloc(c, &d.epsSum, (size_t)L * A)) || (rc = dalloc(c, &d.FstSum, K)) ||
+        (rc = dalloc(c, &d.alphaSum, K)) || (rc = dalloc(c, &d.lambdaSum, K)) ||
+        (rc = dalloc(c, &d.like3, 3))) {
+        return rc;
+    }
+    return SGPU_OK;
+}
+
+int upload_descriptors(sgpu_ctx *c)
+{
+    std::vector<ChainDesc> v;
+    for (const HostChain &h : c->chains) {
+        v.push_back(h.d);
+    }
+    HIP_OK(hipMemcpyAsync(c->cds, v.data(), v.size() * sizeof(ChainDesc), hipMemcpyHostToDevice,
+                          c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return SGPU_OK;
+}
+
+int build_plans(sgpu_ctx *c)
+{
+    int rc;
+    auto L_K = [](const HostChain &h) { return std::make_pair((int)cdiv((size_t)h.m.L * h.m.K, kNT), 1); };
+    auto one = [](const HostChain &) { return std::make_pair(1, 1); };
+    auto qb = [](const HostChain &h) { return std::make_pair(h.d.q_blocks, 1); };
+    auto lb = [](const HostChain &h) { return std::make_pair(h.d.l_blocks, 1); };
+    if ((rc = build_plan(c, c->p_update_p8, [](const HostChain &h) { return h.m.A <= 8; }, L_K)) ||
+        (rc = build_plan(c, c->p_update_p0, [](const HostChain &h) { return h.m.A > 8; }, L_K)) ||
+        (rc = build_plan(c, c->p_fold, all_chains,
+                         [](const HostChain &h) {
+                             return std::make_pair((int)cdiv((size_t)h.m.N * h.m.K, kNT), kRowSplit);
+                         })) ||
+        (rc = build_plan(c, c->p_gibbs, all_chains,
+                         [](const HostChain &h) { return std::make_pair((int)cdiv(h.m.N, kGibbsInds), 1); })) ||
+        (rc = build_plan(c, c->p_logq, all_chains, qb)) ||
+        (rc = build_plan(c, c->p_accept, all_chains, qb)) ||
+        (rc = build_plan(c, c->p_alpha, [](const HostChain &h) { return h.m.inferalpha != 0; }, one)) ||
+        (rc = build_plan(c, c->p_lam_part, [](const HostChain &h) { return h.m.inferlambda != 0; }, lb)) ||
+        (rc = build_plan(c, c->p_lam_fin, [](const HostChain &h) { return h.m.inferlambda != 0; }, one)) ||
+        (rc = build_plan(c, c->p_eps, [](const HostChain &h) { return h.m.freqscorr != 0; },
+                         [](const HostChain &h) { return std::make_pair((int)cdiv(h.m.L, kNT / h.m.K), 1); })) ||
+        (rc = build_plan(c, c->p_fst_part, [](const HostChain &h) { return h.m.freqscorr != 0; }, lb)) ||
+        (rc = build_plan(c, c->p_fst_fin, [](const HostChain &h) { return h.m.freqscorr != 0; }, one)) ||
+        (rc = build_plan(c, c->p_collect, all_chains, one)) ||
+        (rc = build_plan(c, c->p_init_eps, all_chains, lb))) {
+        return rc;
+    }
+    c->p_logq_chain.resize(c->chains.size());
+    for (size_t ci = 0; ci < c->chains.size(); ci++) {
+        const HostChain *target = &c->chains[ci];
+        if ((rc = build_plan(c, c->p_logq_chain[ci],
+                             [target](const HostChain &h) { return &h == target; }, qb))) {
+            return rc;
+        }
+    }
+    // K-templated launches: one per (format, K), heavier K first
+    std::map<std::pair<int, int>, int> keys;
+    for (const HostChain &h : c->chains) {
+        keys[{-h.d.fmt, -h.m.K}] = 1;
+    }
+    for (auto &kv : keys) {
+        const int fmt = -kv.first.first, K = -kv.first.second;
+        auto sel = [fmt, K](const HostChain &h) { return h.d.fmt == fmt && h.m.K == K; };
+        KGroup zg;
+        zg.fmt = fmt;
+        zg.K = K;
+        if ((rc = build_plan(c, zg.plan, sel,
+                             [](const HostChain &h) { return std::make_pair(h.d.n_lt, h.d.n_it); }))) {
+            return rc;
+        }
+        for (const HostChain &h : c->chains) {
+            if (sel(h)) {
+                zg.plan.lds = std::max(zg.plan.lds, h.z_lds);
+            }
+        }
+        c->z_groups.push_back(zg);
+        KGroup mg;
+        mg.fmt = fmt;
+        mg.K = K;
+        if ((rc = build_plan(c, mg.plan, sel,
+                             [](const HostChain &h) { return std::make_pair(h.d.q_blocks, h.d.n_ranges); }))) {
+            return rc;
+        }
+        c->metro_groups.push_back(mg);
+    }
+    // byte chains of different K share one runtime-K kernel, but each group still
+    // launches separately; that keeps the LDS size per launch tight
+    return SGPU_OK;
+}
+
+int chain_index(sgpu_ctx *c, int32_t chain)
+{
+    if (!c) {
+        return fail(SGPU_ERR_ARG, "null context");
+    }
+    if (chain < 0 || (size_t)chain >= c->chains.size()) {
+        return fail(SGPU_ERR_ARG, "chain %d outside [0, %zu)", chain, c->chains.size());
+    }
+    return SGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sgpu_last_error(void) { return g_err; }
+
+const char *sgpu_version(void)
+{
+    return "sgpu 0.2 gfx950: chain-batched launches, fused UpdateZ (SNP2 and byte formats, K<=16), "
+           "FP64 draws";
+}
+
+int sgpu_create(const sgpu_model *m, const sgpu_data *d, sgpu_ctx **out)
+{
+    const int32_t zero = 0;
+    return sgpu_create_batch(m, 1, d, 1, &zero, nullptr, out);
+}
+
+int sgpu_create_batch(const sgpu_model *models, int32_t nchains, const sgpu_data *data,
+                      int32_t ndata, const int32_t *data_of_chain, const sgpu_tuning *tuning,
+                      sgpu_ctx **out)
+{
+    if (!out) {
+        return fail(SGPU_ERR_ARG, "null argument");
+    }
+    *out = nullptr;
+    if (!models || !data || !data_of_chain || nchains < 1 || ndata < 1) {
+        return fail(SGPU_ERR_ARG, "null argument or empty batch");
+    }
+    for (int ci = 0; ci < nchains; ci++) {
+        int rc = check_model(&models[ci], ci);
+        if (rc) {
+            return rc;
+        }
+        const int di = data_of_chain[ci];
+        if (di < 0 || di >= ndata) {
+            return fail(SGPU_ERR_ARG, "chain %d names data set %d of %d", ci, di, ndata);
+        }
+        const sgpu_model &m0 = models[0];
+        const sgpu_model &mc = models[ci];
+        if (mc.burnin != m0.burnin || mc.metrofreq != m0.metrofreq ||
+            mc.computeprob != m0.computeprob) {
+            return fail(SGPU_ERR_ARG,
+                        "chain %d: BURNIN, METROFREQ and COMPUTEPROB must match across a batch", ci);
+        }
+    }
+    // each data set's shape comes from its first chain; later chains must agree
+    std::vector<int> first(ndata, -1);
+    for (int ci = 0; ci < nchains; ci++) {
+        const int di = data_of_chain[ci];
+        if (first[di] < 0) {
+            first[di] = ci;
+            int rc = check_data(&data[di], models[ci].N, models[ci].L, models[ci].A, di);
+            if (rc) {
+                return rc;
+            }
+        } else {
+            const sgpu_model &a = models[first[di]], &b = models[ci];
+            if (a.N != b.N || a.L != b.L || a.A != b.A) {
+                return fail(SGPU_ERR_ARG, "chain %d: N, L, A differ from chain %d on data set %d",
+                            ci, first[di], di);
+            }
+        }
+    }
+    const int z_target = (tuning && tuning->z_blocks > 0) ? tuning->z_blocks : kZBlocksDefault;
+    const int metro_target =
+        (tuning && tuning->metro_blocks > 0) ? tuning->metro_blocks : kMetroBlocksDefault;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        return fail(SGPU_ERR_DEVICE, "no HIP device available");
+    }
+
+    sgpu_ctx *c = new sgpu_ctx();
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        delete c;
+        return fail(SGPU_ERR_DEVICE, "hipGetDevice failed");
+    }
+    auto cleanup = [&](int code) {
+        sgpu_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(SGPU_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    c->burnin = models[0].burnin;
+    c->metrofreq = models[0].metrofreq;
+    c->computeprob = models[0].computeprob;
+
+    int rc;
+    c->data.resize(ndata);
+    for (int di = 0; di < ndata; di++) {
+        if (first[di] < 0) {
+            continue;   // no chain uses it
+        }
+        const sgpu_model &m = models[first[di]];
+        Dataset &ds = c->data[di];
+        ds.fmt = data[di].format;
+        ds.N = m.N;
+        ds.L = m.L;
+        ds.A = m.A;
+        ds.G = (int)cdiv(m.N, ds.fmt == SGPU_GENO_SNP2 ? 16 : 2);
+        ds.Lpad = (int)cdiv(m.L, kZLoci) * kZLoci;
+        if ((rc = upload_dataset(c, ds, &data[di]))) {
+            return cleanup(rc);
+        }
+    }
+    int total_lt = 0, total_qb = 0;
+    for (int ci = 0; ci < nchains; ci++) {
+        total_lt += (int)cdiv(models[ci].L, kZLoci);
+        total_qb += (int)cdiv(models[ci].N, kNT);
+    }
+    c->chains.resize(nchains);
+    std::vector<double> lam0(nchains);
+    for (int ci = 0; ci < nchains; ci++) {
+        HostChain &h = c->chains[ci];
+        h.m = models[ci];
+        h.dataset = data_of_chain[ci];
+        lam0[ci] = h.m.lambda0;
+        c->any_alpha |= h.m.inferalpha != 0;
+        c->any_lambda |= h.m.inferlambda != 0;
+        c->any_freqscorr |= h.m.freqscorr != 0;
+        if ((rc = setup_chain(c, h, c->data[h.dataset], total_lt, total_qb, z_target,
+                              metro_target))) {
+            return cleanup(rc);
+        }
+    }
+    if ((rc = dalloc(c, &c->cds, nchains)) || (rc = dalloc(c, &c->lambda0, nchains)) ||
+        (rc = dalloc(c, &c->repd, 2)) || (rc = upload_descriptors(c)) || (rc = build_plans(c))) {
+        return cleanup(rc);
+    }
+    hipError_t e = hipMemcpyAsync(c->lambda0, lam0.data(), nchains * sizeof(double),
+                                  hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
         e = hipStreamSynchronize(c->stream);
     }
@@ -582,6 +840,8 @@ int sgpu_create(const sgpu_model *m, const sgpu_data *d, sgpu_ctx **out)
     *out = c;
     return SGPU_OK;
 }
+
+int sgpu_num_chains(const sgpu_ctx *c) { return c ? (int)c->chains.size() : 0; }
 
 int sgpu_destroy(sgpu_ctx *c)
 {
@@ -621,39 +881,36 @@ int sgpu_init_state(sgpu_ctx *c)
     if (!c) {
         return fail(SGPU_ERR_ARG, "null context");
     }
-    const sgpu_model &m = c->m;
-    const int N = m.N, L = m.L, K = m.K, A = m.A;
-    std::vector<double> kv(K), q((size_t)N * K, 1.0 / K);
-    for (int k = 0; k < K; k++) {
-        kv[k] = m.alpha0;
+    for (HostChain &h : c->chains) {
+        const sgpu_model &m = h.m;
+        const ChainDesc &d = h.d;
+        const int N = m.N, L = m.L, K = m.K, A = m.A;
+        std::vector<double> alpha(K, m.alpha0), lambda(K, m.lambda0),
+            F(K, m.freqscorr ? m.fpriormean : 0.5), q((size_t)N * K, 1.0 / K);
+        HIP_OK(hipMemcpyAsync(d.alpha, alpha.data(), K * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        HIP_OK(hipMemcpyAsync(d.lambda, lambda.data(), K * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        HIP_OK(hipMemcpyAsync(d.F, F.data(), K * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(d.Q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        HIP_OK(hipMemsetAsync(d.P, 0, (size_t)L * K * A * sizeof(double), c->stream));
+        HIP_OK(hipMemsetAsync(d.QSum, 0, (size_t)N * K * sizeof(double), c->stream));
+        HIP_OK(hipMemsetAsync(d.PSum, 0, (size_t)L * K * A * sizeof(double), c->stream));
+        HIP_OK(hipMemsetAsync(d.epsSum, 0, (size_t)L * A * sizeof(double), c->stream));
+        HIP_OK(hipMemsetAsync(d.FstSum, 0, K * sizeof(double), c->stream));
+        HIP_OK(hipMemsetAsync(d.alphaSum, 0, K * sizeof(double), c->stream));
+        HIP_OK(hipMemsetAsync(d.lambdaSum, 0, K * sizeof(double), c->stream));
+        HIP_OK(hipMemsetAsync(d.like3, 0, 3 * sizeof(double), c->stream));
+        // the host vectors die at the end of this iteration
+        HIP_OK(hipStreamSynchronize(c->stream));
     }
-    HIP_OK(hipMemcpyAsync(c->alpha, kv.data(), K * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    for (int k = 0; k < K; k++) {
-        kv[k] = m.lambda0;
-    }
-    HIP_OK(hipMemcpyAsync(c->lambda, kv.data(), K * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    for (int k = 0; k < K; k++) {
-        kv[k] = m.freqscorr ? m.fpriormean : 0.5;
-    }
-    HIP_OK(hipMemcpyAsync(c->F, kv.data(), K * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(c->Q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemsetAsync(c->P, 0, (size_t)L * K * A * sizeof(double), c->stream));
-    HIP_OK(hipMemsetAsync(c->QSum, 0, (size_t)N * K * sizeof(double), c->stream));
-    HIP_OK(hipMemsetAsync(c->PSum, 0, (size_t)L * K * A * sizeof(double), c->stream));
-    HIP_OK(hipMemsetAsync(c->epsSum, 0, (size_t)L * A * sizeof(double), c->stream));
-    HIP_OK(hipMemsetAsync(c->FstSum, 0, K * sizeof(double), c->stream));
-    HIP_OK(hipMemsetAsync(c->alphaSum, 0, K * sizeof(double), c->stream));
-    HIP_OK(hipMemsetAsync(c->lambdaSum, 0, K * sizeof(double), c->stream));
-    HIP_OK(hipMemsetAsync(c->like3, 0, 3 * sizeof(double), c->stream));
-    hipLaunchKernelGGL(k_init_eps, dim3(cdiv(L, 256)), dim3(256), 0, c->stream, c->ma, c->fmt,
-                       c->geno, c->Lpad, c->nall, m.lambda0, c->eps);
-    LAUNCH_OK();
-    c->rep_known = false;
-    int rc = use_rep(c, 0);
-    if (rc || (rc = launch_z(c, 0, 0, nullptr))) {
+    int rc = launch(c, k_init_eps, c->p_init_eps, kNT, (const double *)c->lambda0);
+    if (rc) {
         return rc;
     }
-    if ((rc = launch_logq(c))) {
+    c->rep_known = false;
+    if ((rc = use_rep(c, 0)) || (rc = launch_z(c, 0)) || (rc = launch(c, k_logq_stats, c->p_logq, kNT))) {
         return rc;
     }
     c->nsum = 0;
@@ -662,13 +919,18 @@ int sgpu_init_state(sgpu_ctx *c)
     return SGPU_OK;
 }
 
-int sgpu_get_state(sgpu_ctx *c, int32_t which, void *host, size_t bytes)
+int sgpu_get_chain_state(sgpu_ctx *c, int32_t chain, int32_t which, void *host, size_t bytes)
 {
-    if (!c || !host) {
+    int rc = chain_index(c, chain);
+    if (rc) {
+        return rc;
+    }
+    if (!host) {
         return fail(SGPU_ERR_ARG, "null argument");
     }
+    HostChain &h = c->chains[chain];
     size_t elem;
-    const size_t n = state_elems(c, which, &elem);
+    const size_t n = state_elems(h.m, which, &elem);
     if (n == 0) {
         return fail(SGPU_ERR_ARG, "unknown state selector %d", which);
     }
@@ -680,12 +942,12 @@ int sgpu_get_state(sgpu_ctx *c, int32_t which, void *host, size_t bytes)
         memcpy(host, &c->nsum, sizeof(int64_t));
         return SGPU_OK;
     }
-    const void *src = state_ptr(c, which);
+    const void *src = state_ptr(h.d, which);
     if (which == SGPU_NA || which == SGPU_NQ) {
         const bool isA = which == SGPU_NA;
-        int32_t *dst = isA ? c->nA : c->nQ;
+        int32_t *dst = isA ? h.nA : h.nQ;
         hipLaunchKernelGGL(k_reduce_counts, dim3(cdiv(n, 256)), dim3(256), 0, c->stream,
-                           isA ? c->nApart : c->nQpart, isA ? c->n_it : c->n_lt, n, dst);
+                           isA ? h.d.nApart : h.d.nQpart, isA ? h.d.n_it : h.d.n_lt, n, dst);
         LAUNCH_OK();
         src = dst;
     }
@@ -694,13 +956,19 @@ int sgpu_get_state(sgpu_ctx *c, int32_t which, void *host, size_t bytes)
     return SGPU_OK;
 }
 
-int sgpu_set_state(sgpu_ctx *c, int32_t which, const void *host, size_t bytes)
+int sgpu_set_chain_state(sgpu_ctx *c, int32_t chain, int32_t which, const void *host,
+                         size_t bytes)
 {
-    if (!c || !host) {
+    int rc = chain_index(c, chain);
+    if (rc) {
+        return rc;
+    }
+    if (!host) {
         return fail(SGPU_ERR_ARG, "null argument");
     }
+    HostChain &h = c->chains[chain];
     size_t elem;
-    const size_t n = state_elems(c, which, &elem);
+    const size_t n = state_elems(h.m, which, &elem);
     if (n == 0) {
         return fail(SGPU_ERR_ARG, "unknown state selector %d", which);
     }
@@ -713,19 +981,29 @@ int sgpu_set_state(sgpu_ctx *c, int32_t which, const void *host, size_t bytes)
     }
     if (which == SGPU_NA || which == SGPU_NQ) {
         const bool isA = which == SGPU_NA;
-        int32_t *part = isA ? c->nApart : c->nQpart;
-        const int nparts = isA ? c->n_it : c->n_lt;
+        int32_t *part = isA ? h.d.nApart : h.d.nQpart;
+        const int nparts = isA ? h.d.n_it : h.d.n_lt;
         HIP_OK(hipMemsetAsync(part, 0, (size_t)nparts * n * sizeof(int32_t), c->stream));
         HIP_OK(hipMemcpyAsync(part, host, bytes, hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
         return SGPU_OK;
     }
-    HIP_OK(hipMemcpyAsync(state_ptr(c, which), host, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(hipMemcpyAsync(state_ptr(h.d, which), host, bytes, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     if (which == SGPU_Q) {
-        return launch_logq(c);
+        return launch(c, k_logq_stats, c->p_logq_chain[chain], kNT);
     }
     return SGPU_OK;
+}
+
+int sgpu_get_state(sgpu_ctx *c, int32_t which, void *host, size_t bytes)
+{
+    return sgpu_get_chain_state(c, 0, which, host, bytes);
+}
+
+int sgpu_set_state(sgpu_ctx *c, int32_t which, const void *host, size_t bytes)
+{
+    return sgpu_set_chain_state(c, 0, which, host, bytes);
 }
 
 int sgpu_update_p(sgpu_ctx *c, uint32_t rep, int32_t collect)
@@ -735,17 +1013,11 @@ int sgpu_update_p(sgpu_ctx *c, uint32_t rep, int32_t collect)
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_P);
-    const size_t nthreads = (size_t)c->m.L * c->m.K;
-    if (c->m.A <= 8) {
-        hipLaunchKernelGGL(k_update_p<8>, dim3(cdiv(nthreads, kNT)), dim3(kNT), 0, c->stream, c->ma,
-                           c->nall, c->nApart, c->n_it, c->eps, c->F, c->lambda, c->P, c->PSum,
-                           (int)collect, (const uint32_t *)c->repd);
-    } else {
-        hipLaunchKernelGGL(k_update_p<0>, dim3(cdiv(nthreads, kNT)), dim3(kNT), 0, c->stream, c->ma,
-                           c->nall, c->nApart, c->n_it, c->eps, c->F, c->lambda, c->P, c->PSum,
-                           (int)collect, (const uint32_t *)c->repd);
+    const uint32_t *r = c->repd;
+    if ((rc = launch(c, k_update_p<8>, c->p_update_p8, kNT, r, (int)collect)) ||
+        (rc = launch(c, k_update_p<0>, c->p_update_p0, kNT, r, (int)collect))) {
+        return rc;
     }
-    LAUNCH_OK();
     return SGPU_OK;
 }
 
@@ -756,37 +1028,24 @@ int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_Q);
-    const bool metro = c->m.metrofreq > 0 && rep % (uint32_t)c->m.metrofreq == 0;
+    const uint32_t *r = c->repd;
+    const bool metro = c->metrofreq > 0 && rep % (uint32_t)c->metrofreq == 0;
     if (metro) {
-        dim3 grid(c->q_blocks, c->n_ranges);
-        if (c->fmt == SGPU_GENO_SNP2) {
-            mkern_t k = metro_snp2_kernel(c->m.K);
+        for (const KGroup &g : c->metro_groups) {
+            mkern_t k = metro_kernel(g.fmt, g.K);
             if (!k) {
-                return fail(SGPU_ERR_UNSUPPORTED, "no SNP2 Metropolis kernel for K=%d", c->m.K);
+                return fail(SGPU_ERR_UNSUPPORTED, "no Metropolis kernel for K=%d", g.K);
             }
-            hipLaunchKernelGGL(k, grid, dim3(kNT), 0, c->stream, c->ma, c->geno, c->Lpad, c->Q,
-                               c->P, c->alpha, c->loci_per_range, c->delta_part, (const uint32_t *)c->repd);
-        } else {
-            hipLaunchKernelGGL(k_metro_ratio_byte, grid, dim3(kNT), 0, c->stream, c->ma, c->geno,
-                               c->Lpad, c->Q, c->P, c->alpha, c->loci_per_range, c->delta_part, (const uint32_t *)c->repd);
+            if ((rc = launch(c, k, g.plan, kNT, r))) {
+                return rc;
+            }
         }
-        LAUNCH_OK();
-        hipLaunchKernelGGL(k_metro_accept, dim3(c->q_blocks), dim3(kNT), 0, c->stream, c->ma,
-                           c->n_ranges, c->delta_part, c->alpha, c->Q, c->QSum, (int)collect,
-                           c->logq_part, (int32_t *)nullptr, (const uint32_t *)c->repd);
-        c->logq_rows = c->q_blocks;
-    } else {
-        const size_t nk = (size_t)c->m.N * c->m.K;
-        hipLaunchKernelGGL(k_fold_rows, dim3(cdiv(nk, kNT), kRowSplit), dim3(kNT), 0, c->stream,
-                           c->nQpart, c->n_lt, nk, c->nQfold);
-        LAUNCH_OK();
-        hipLaunchKernelGGL(k_update_q_gibbs, dim3(cdiv(c->m.N, kGibbsInds)), dim3(kGibbsInds), 0,
-                           c->stream, c->ma, c->nQfold, c->alpha, c->Q, c->QSum, (int)collect,
-                           c->logq_part, (const uint32_t *)c->repd);
-        c->logq_rows = (int)cdiv(c->m.N, kGibbsInds);
+        return launch(c, k_metro_accept, c->p_accept, kNT, r, (int)collect);
     }
-    LAUNCH_OK();
-    return SGPU_OK;
+    if ((rc = launch(c, k_fold_rows, c->p_fold, kNT))) {
+        return rc;
+    }
+    return launch(c, k_update_q_gibbs, c->p_gibbs, kGibbsInds, r, (int)collect);
 }
 
 int sgpu_update_z(sgpu_ctx *c, uint32_t rep, int32_t flags, int8_t *zout)
@@ -795,24 +1054,31 @@ int sgpu_update_z(sgpu_ctx *c, uint32_t rep, int32_t flags, int8_t *zout)
     if (rc || (rc = use_rep(c, rep))) {
         return rc;
     }
+    if (zout && c->chains.size() != 1) {
+        return fail(SGPU_ERR_ARG, "a debug copy of Z needs a single-chain context");
+    }
     PhaseTimer timer(c, SGPU_PH_Z);
-    int8_t *zdev = nullptr;
-    const size_t zn = (size_t)c->m.N * 2 * c->m.L;
+    const bool like = (flags & SGPU_Z_LIKE) != 0;
     if (zout) {
+        HostChain &h = c->chains[0];
+        const size_t zn = (size_t)h.m.N * 2 * h.m.L;
         if (!c->zdbg) {
             HIP_OK(hipMalloc(&c->zdbg, zn));
         }
-        zdev = c->zdbg;
-    }
-    const bool like = (flags & SGPU_Z_LIKE) != 0;
-    if ((rc = launch_z(c, rep, like ? 2 : 1, zdev))) {
+        h.d.zout = c->zdbg;
+        if ((rc = upload_descriptors(c)) || (rc = launch_z(c, like ? 4 : 3))) {
+            return rc;
+        }
+        h.d.zout = nullptr;
+        if ((rc = upload_descriptors(c))) {
+            return rc;
+        }
+        HIP_OK(hipMemcpyAsync(zout, c->zdbg, zn, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    } else if ((rc = launch_z(c, like ? 2 : 1))) {
         return rc;
     }
     c->like_ready = like;
-    if (zout) {
-        HIP_OK(hipMemcpyAsync(zout, zdev, zn, hipMemcpyDeviceToHost, c->stream));
-        HIP_OK(hipStreamSynchronize(c->stream));
-    }
     return SGPU_OK;
 }
 
@@ -823,10 +1089,7 @@ int sgpu_update_alpha(sgpu_ctx *c, uint32_t rep)
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_ALPHA);
-    hipLaunchKernelGGL(k_update_alpha, dim3(1), dim3(kNT), 0, c->stream, c->ma, c->logq_part,
-                       c->logq_rows, c->alpha, (const uint32_t *)c->repd);
-    LAUNCH_OK();
-    return SGPU_OK;
+    return launch(c, k_update_alpha, c->p_alpha, kNT, (const uint32_t *)c->repd);
 }
 
 int sgpu_update_lambda(sgpu_ctx *c, uint32_t rep)
@@ -836,13 +1099,10 @@ int sgpu_update_lambda(sgpu_ctx *c, uint32_t rep)
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_LAMBDA);
-    hipLaunchKernelGGL(k_lambda_partial, dim3(c->l_blocks), dim3(kNT), 0, c->stream, c->ma, c->nall,
-                       c->P, c->eps, c->lam_part);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(k_lambda_finalize, dim3(1), dim3(kNT), 0, c->stream, c->ma, c->lam_part,
-                       c->l_blocks, c->ahist, c->nhist, c->lambda, (const uint32_t *)c->repd);
-    LAUNCH_OK();
-    return SGPU_OK;
+    if ((rc = launch(c, k_lambda_partial, c->p_lam_part, kNT))) {
+        return rc;
+    }
+    return launch(c, k_lambda_finalize, c->p_lam_fin, kNT, (const uint32_t *)c->repd);
 }
 
 int sgpu_update_eps(sgpu_ctx *c, uint32_t rep, int32_t collect)
@@ -852,11 +1112,7 @@ int sgpu_update_eps(sgpu_ctx *c, uint32_t rep, int32_t collect)
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_EPS);
-    hipLaunchKernelGGL(k_update_eps, dim3(cdiv(c->m.L, kNT / c->m.K)), dim3(kNT), 0, c->stream,
-                       c->ma, c->nall, c->P,
-                       c->F, c->lambda, c->eps, c->epsSum, (int)collect, (const uint32_t *)c->repd);
-    LAUNCH_OK();
-    return SGPU_OK;
+    return launch(c, k_update_eps, c->p_eps, kNT, (const uint32_t *)c->repd, (int)collect);
 }
 
 int sgpu_update_fst(sgpu_ctx *c, uint32_t rep)
@@ -866,13 +1122,11 @@ int sgpu_update_fst(sgpu_ctx *c, uint32_t rep)
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_FST);
-    hipLaunchKernelGGL(k_fst_partial, dim3(c->l_blocks), dim3(kNT), 0, c->stream, c->ma, c->nall,
-                       c->eps, c->P, c->F, c->fst_part, (const uint32_t *)c->repd);
-    LAUNCH_OK();
-    hipLaunchKernelGGL(k_fst_finalize, dim3(1), dim3(kNT), 0, c->stream, c->ma, c->fst_part,
-                       c->l_blocks, c->nloci_nonzero, c->F, (const uint32_t *)c->repd);
-    LAUNCH_OK();
-    return SGPU_OK;
+    const uint32_t *r = c->repd;
+    if ((rc = launch(c, k_fst_partial, c->p_fst_part, kNT, r))) {
+        return rc;
+    }
+    return launch(c, k_fst_finalize, c->p_fst_fin, kNT, r);
 }
 
 int sgpu_collect(sgpu_ctx *c, uint32_t rep)
@@ -883,11 +1137,10 @@ int sgpu_collect(sgpu_ctx *c, uint32_t rep)
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_COLLECT);
-    const int with_like = (c->m.computeprob && c->like_ready) ? 1 : 0;
-    hipLaunchKernelGGL(k_collect, dim3(1), dim3(kNT), 0, c->stream, c->ma, c->likepart,
-                       c->n_it * c->n_lt, with_like, c->alpha, c->lambda, c->F, c->alphaSum,
-                       c->lambdaSum, c->FstSum, c->like3);
-    LAUNCH_OK();
+    const int with_like = (c->computeprob && c->like_ready) ? 1 : 0;
+    if ((rc = launch(c, k_collect, c->p_collect, kNT, with_like))) {
+        return rc;
+    }
     c->nsum++;
     return SGPU_OK;
 }
@@ -898,15 +1151,14 @@ int sgpu_sweep(sgpu_ctx *c, uint32_t rep)
     if (rc) {
         return rc;
     }
-    const sgpu_model &m = c->m;
-    const int collect = ((int64_t)rep + 1 > m.burnin) ? 1 : 0;
+    const int collect = ((int64_t)rep + 1 > c->burnin) ? 1 : 0;
     if ((rc = sgpu_update_p(c, rep, collect))) return rc;
     if ((rc = sgpu_update_q(c, rep, collect))) return rc;
-    if ((rc = sgpu_update_z(c, rep, (collect && m.computeprob) ? SGPU_Z_LIKE : 0, nullptr)))
+    if ((rc = sgpu_update_z(c, rep, (collect && c->computeprob) ? SGPU_Z_LIKE : 0, nullptr)))
         return rc;
-    if (m.inferalpha && (rc = sgpu_update_alpha(c, rep))) return rc;
-    if (m.inferlambda && (rc = sgpu_update_lambda(c, rep))) return rc;
-    if (m.freqscorr) {
+    if (c->any_alpha && (rc = sgpu_update_alpha(c, rep))) return rc;
+    if (c->any_lambda && (rc = sgpu_update_lambda(c, rep))) return rc;
+    if (c->any_freqscorr) {
         if ((rc = sgpu_update_eps(c, rep, collect))) return rc;
         if ((rc = sgpu_update_fst(c, rep))) return rc;
     }
@@ -930,14 +1182,13 @@ int sgpu_run_sweeps(sgpu_ctx *c, uint32_t rep0, uint32_t n)
     }
     // One graph per sweep type (Metropolis or Gibbs Q, burn-in or collecting),
     // captured on first use; each replay ends by advancing the device index.
-    const sgpu_model &m = c->m;
     if ((rc = use_rep(c, rep0))) {
         return rc;
     }
     for (uint32_t r = 0; r < n; r++) {
         const uint32_t rep = rep0 + r;
-        const bool metro = m.metrofreq > 0 && rep % (uint32_t)m.metrofreq == 0;
-        const bool collect = (int64_t)rep + 1 > m.burnin;
+        const bool metro = c->metrofreq > 0 && rep % (uint32_t)c->metrofreq == 0;
+        const bool collect = (int64_t)rep + 1 > c->burnin;
         const int type = (metro ? 1 : 0) | (collect ? 2 : 0);
         if (!c->graph[type]) {
             hipGraph_t g = nullptr;
@@ -974,8 +1225,7 @@ int sgpu_run_sweeps(sgpu_ctx *c, uint32_t rep0, uint32_t n)
         if (collect) {
             c->nsum++;
         }
-        c->like_ready = collect && m.computeprob;
-        c->logq_rows = metro ? c->q_blocks : (int)cdiv(m.N, kGibbsInds);
+        c->like_ready = collect && c->computeprob;
     }
     return SGPU_OK;
 }

@@ -1,6 +1,7 @@
 // Internal definitions shared by the sgpu kernels and the C-ABI layer.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace sgpu {
@@ -28,18 +29,63 @@ constexpr int kZLoci = 256;        // loci per UpdateZ block (one locus per lane
 constexpr int kMaxK = 16;          // largest K the kernels take
 constexpr int kMaxAlleles = 255;   // byte format
 
-struct ZArgs {
-    const uint32_t *geno;      // packed words
-    const double *Q;           // [N][K]
-    const double *P;           // [L][K][A]
-    int32_t *nApart;           // [n_it][L][K][A]
-    int32_t *nQpart;           // [n_lt][N][K]
-    double *likepart;          // [n_it][n_lt]
-    int8_t *zout;              // [N][2][L] or null
-    int N, L, K, A, Lpad, G;   // G = number of genotype word rows
-    int rows_per_tile;         // word rows per individual tile
-    int inds_per_row;          // 16 (SNP2) or 2 (BYTE)
+struct ModelArgs {
+    int N, L, K, A;
+    int freqscorr, onefst, popalphas, unifprioralpha, popspecificlambda;
+    int inferalpha, inferlambda, pad0;
+    double alphapropsd, alphamax, alphapriora, alphapriorb, fpriormean, fpriorsd;
     uint32_t seed, chain;
+};
+
+// One chain of a batch as the kernels see it: its model, its tilings and the
+// device addresses of its state.  A context keeps an array of these in device
+// memory; chains may differ in N, L, K, A, data set and flags.
+struct ChainDesc {
+    ModelArgs m;
+    int fmt;                       // 1 = 2-bit SNP words, 2 = byte per copy
+    int Lpad, G;                   // padded loci per word row, word rows
+    int n_lt, n_it, rows_per_tile; // UpdateZ tiles: loci tiles, individual tiles, rows per tile
+    int q_blocks, l_blocks;        // cdiv(N, 256), cdiv(L, 256)
+    int n_ranges, loci_per_range;  // Metropolis locus ranges
+    int nhist, nloci_nonzero;      // allele-count histogram size, loci with alleles
+    const uint32_t *geno;          // packed words [G][Lpad]
+    const int32_t *nall;           // [L]
+    const int32_t *ahist;          // (allele count, loci) pairs
+    double *Q, *P, *eps, *F, *alpha, *lambda;
+    int32_t *nApart;               // [n_it][L][K][A]
+    int32_t *nQpart;               // [n_lt][N][K]
+    int32_t *nQfold;               // [16][N][K]
+    int32_t *logq_rows;            // rows of logq_part written by the last Q update
+    double *likepart;              // [n_it][n_lt]
+    double *logq_part;             // [rows][K+1]
+    double *delta_part;            // [n_ranges][N]
+    double *lam_part, *fst_part;   // [l_blocks][K]
+    double *QSum, *PSum, *epsSum, *FstSum, *alphaSum, *lambdaSum, *like3;
+    int8_t *zout;                  // debug copy of Z [N][2][L] or null
+};
+
+#ifdef __HIP_DEVICE_COMPILE__
+// Global-memory view of a device address read from a descriptor: lets the
+// compiler emit global (not flat) loads and stores.
+#define SGPU_G __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ SGPU_G T *g(T *p)
+{
+    return (SGPU_G T *)p;
+}
+#else
+#define SGPU_G
+template <class T>
+__host__ __device__ inline T *g(T *p)
+{
+    return p;
+}
+#endif
+
+// A block of a batched launch: its chain and its block coordinates within
+// that chain's own grid.
+struct BlockRef {
+    uint32_t chain, bx, by, pad;
 };
 
 }  // namespace sgpu

@@ -1,0 +1,109 @@
+"""Chain batches (sgpu_create_batch): chains of uneven K, flags, seeds and data
+sets advanced by one launch per kernel must follow exactly the path each chain
+follows alone.  Each chain's Philox stream depends only on its own key, and the
+counts are exact integers whatever the tiling, so Z, the counts, P, Q, alpha,
+lambda, eps, F and the posterior sums are bit-identical; lnP(D) is an FP64 sum
+over the UpdateZ tiles, whose number depends on the batch, so it agrees to
+1e-12 relative."""
+import numpy as np
+import pytest
+
+import structure_amd as sa
+
+pytestmark = pytest.mark.gpu
+
+EXACT = (sa.Q, sa.P, sa.EPS, sa.FST, sa.ALPHA, sa.LAMBDA, sa.NA, sa.NQ, sa.QSUM, sa.PSUM,
+         sa.EPSSUM, sa.FSTSUM, sa.ALPHASUM, sa.LAMBDASUM, sa.NSUM)
+
+
+def _datasets():
+    snp = sa.synth_snp(90, 700, Ktrue=4, qalpha=0.5, miss=0.02, seed=21)
+    ms = sa.synth_microsat(60, 300, amin=2, amax=7, Ktrue=3, miss=0.03, seed=22)
+    fmt, words = sa.pack(ms["geno"], sa.GENO_BYTE)
+    return [(sa.GENO_SNP2, snp["words"], snp["nall"], 90, 700, 2),
+            (fmt, words, ms["nall"], 60, 300, ms["A"])]
+
+
+CHAINS = [  # (data set, K, options)
+    (0, 1, dict(seed=1, chain=0)),
+    (0, 2, dict(seed=1, chain=1, inferlambda=1)),
+    (0, 5, dict(seed=7, chain=0, popalphas=1)),
+    (0, 3, dict(seed=3, chain=4, freqscorr=0, inferlambda=1, popspecificlambda=1)),
+    (0, 5, dict(seed=8, chain=2, onefst=1)),
+    (1, 3, dict(seed=5, chain=0, freqscorr=0, inferalpha=1)),
+    (1, 4, dict(seed=6, chain=1, inferlambda=1)),
+]
+
+
+def _models(dsets, common):
+    out = []
+    for di, K, kw in CHAINS:
+        _, _, _, N, L, A = dsets[di]
+        o = dict(common)
+        o.update(kw)
+        out.append(sa.make_model(N, L, K, A, **o))
+    return out
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_batch_equals_chains_run_alone(graphs):
+    dsets = _datasets()
+    common = dict(burnin=6, metrofreq=4)
+    models = _models(dsets, common)
+    batch = sa.Batch(models, [d[:3] for d in dsets], [c[0] for c in CHAINS])
+    assert len(batch) == len(CHAINS)
+    batch.init()
+    nsweeps = 17
+    if graphs:
+        batch.run(0, 9)
+        batch.run(9, nsweeps - 9)
+    else:
+        for r in range(nsweeps):
+            batch.sweep(r)
+    for ci, (di, K, kw) in enumerate(CHAINS):
+        fmt, words, nall = dsets[di][:3]
+        alone = sa.Chain(_models(dsets, common)[ci], fmt, words, nall)
+        alone.init()
+        alone.run(0, nsweeps)
+        for sel in EXACT:
+            assert np.array_equal(batch.get_chain(ci, sel), alone.get(sel)), (ci, sel)
+        lb, la = batch.get_chain(ci, sa.LIKE), alone.get(sa.LIKE)
+        np.testing.assert_allclose(lb, la, rtol=1e-12)
+        assert la[1] != 0.0
+        alone.close()
+    batch.close()
+
+
+def test_batch_tuning_changes_tiles_not_results():
+    dsets = _datasets()
+    common = dict(burnin=3, metrofreq=3)
+    models = _models(dsets, common)
+    a = sa.Batch(models, [d[:3] for d in dsets], [c[0] for c in CHAINS])
+    b = sa.Batch(models, [d[:3] for d in dsets], [c[0] for c in CHAINS],
+                 tuning=sa.Tuning(z_blocks=1, metro_blocks=1))
+    for x in (a, b):
+        x.init()
+        x.run(0, 8)
+    for ci in range(len(CHAINS)):
+        for sel in EXACT:
+            assert np.array_equal(a.get_chain(ci, sel), b.get_chain(ci, sel)), (ci, sel)
+        np.testing.assert_allclose(a.get_chain(ci, sa.LIKE), b.get_chain(ci, sa.LIKE),
+                                   rtol=1e-12)
+
+
+def test_batch_state_is_per_chain():
+    dsets = _datasets()
+    models = _models(dsets, dict(burnin=0, metrofreq=0))
+    batch = sa.Batch(models, [d[:3] for d in dsets], [c[0] for c in CHAINS])
+    batch.init()
+    rng = np.random.default_rng(3)
+    K2 = models[2].K
+    q = rng.dirichlet(np.ones(K2), size=models[2].N)
+    batch.set_chain(2, sa.Q, q)
+    assert np.array_equal(batch.get_chain(2, sa.Q), q)
+    # other chains keep Q = 1/K
+    for ci in (0, 1, 3):
+        assert np.all(batch.get_chain(ci, sa.Q) == 1.0 / models[ci].K)
+    out = np.empty(4)
+    rc = sa.lib().sgpu_get_chain_state(batch._h, len(CHAINS), sa.Q, out.ctypes.data, out.nbytes)
+    assert rc == sa.ERR_ARG

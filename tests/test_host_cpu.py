@@ -141,3 +141,19 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_create_batch_checks_arguments_before_touching_the_device():
+    d = sa.synth_snp(20, 30, Ktrue=2, seed=1)
+    m0 = sa.make_model(20, 30, 2, 2, burnin=5)
+    m1 = sa.make_model(20, 30, 3, 2, burnin=6)
+    with pytest.raises(sa.SgpuError) as e:          # BURNIN differs across the batch
+        sa.Batch([m0, m1], [(sa.GENO_SNP2, d["words"], d["nall"])])
+    assert e.value.code == sa.ERR_ARG
+    m2 = sa.make_model(21, 30, 3, 2, burnin=5)
+    with pytest.raises(sa.SgpuError) as e:          # shares a data set with another N
+        sa.Batch([m0, m2], [(sa.GENO_SNP2, d["words"], d["nall"])])
+    assert e.value.code == sa.ERR_ARG
+    with pytest.raises(sa.SgpuError) as e:          # data set index out of range
+        sa.Batch([m0], [(sa.GENO_SNP2, d["words"], d["nall"])], data_of_chain=[1])
+    assert e.value.code == sa.ERR_ARG
