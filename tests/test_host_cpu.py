@@ -17,10 +17,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(sgpu_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:sgpu|shost)_\w+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header", ["sgpu.h", "sgpu_synth.h"])
+@pytest.mark.parametrize("header", sorted(os.listdir(os.path.join(ROOT, "include"))))
 def test_library_exports_every_declared_symbol(header):
     lib = ctypes.CDLL(sa.LIB_PATH)
     names = _declared(header)
@@ -157,3 +157,22 @@ def test_create_batch_checks_arguments_before_touching_the_device():
     with pytest.raises(sa.SgpuError) as e:          # data set index out of range
         sa.Batch([m0], [(sa.GENO_SNP2, d["words"], d["nall"])], data_of_chain=[1])
     assert e.value.code == sa.ERR_ARG
+
+
+def test_host_struct_layouts_match_the_headers(tmp_path):
+    """The ctypes mirrors of shost_params / shost_input / shost_results have
+    the C layout (sizes and two field offsets, from a program compiled here)."""
+    import ctypes as C
+    from structure_amd import host
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "structure_host.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu", sizeof(shost_params),'
+                   ' offsetof(shost_params, seed), offsetof(shost_params, emerror),'
+                   ' sizeof(shost_input), sizeof(shost_results)); return 0;}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    want = [C.sizeof(host.Params), host.Params.seed.offset, host.Params.emerror.offset,
+            C.sizeof(host.Input), C.sizeof(host.Results)]
+    assert [int(x) for x in got] == want
