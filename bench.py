@@ -37,12 +37,12 @@ def sweep_bytes(N, L, K, A):
     return z_bytes_per_launch(N, L, K, A) + L * K * A * (4 + 8) + L * A * 8
 
 
-def cpu_baseline(N, L, K, sample_loci, sweeps, seed):
+def cpu_baseline(N, L, K, sample_loci, sweeps, seed, ktrue=5, flo=0.01, fhi=0.1, qalpha=0.3):
     import numpy as np
 
     import oracle
     import structure_amd as sa
-    d = sa.synth_snp(N, sample_loci, Ktrue=5, flo=0.01, fhi=0.1, qalpha=0.3, miss=0.01,
+    d = sa.synth_snp(N, sample_loci, Ktrue=ktrue, flo=flo, fhi=fhi, qalpha=qalpha, miss=0.01,
                      seed=seed, want_geno=True)
     m = oracle.make_model(N, sample_loci, K, 2, freqscorr=1, inferalpha=1, inferlambda=1,
                           metrofreq=10, burnin=0, seed=seed, chain=0)
@@ -103,6 +103,194 @@ def batched_ksweep(d, N, L, seed, sweeps, warmup, kmax=10, reps=10):
                 sum(N * L * m.K for m in models) * sweeps / t_batch, 1)}
 
 
+def _timed_run(ch, warmup, steps):
+    """Warm-up sweeps (burn-in), then `steps` timed sweeps from graph replay;
+    returns seconds."""
+    ch.init()
+    ch.run(0, warmup)
+    ch.synchronize()
+    t0 = time.perf_counter()
+    ch.run(warmup, steps)
+    ch.synchronize()
+    return time.perf_counter() - t0
+
+
+def _z_roofline(ch, start, n, zbytes):
+    """Average UpdateZ launch time over n directly launched sweeps (HIP events
+    on the context stream) and the algorithmic-byte HBM fraction."""
+    ch.set_timing(True)
+    ch.get_timing()
+    ch.run(start, n)
+    t = ch.get_timing()
+    ch.set_timing(False)
+    zms, zn = t["Z"]
+    z_avg = zms / max(zn, 1) / 1e3
+    ach = zbytes / z_avg / 1e9
+    return t, {"bound": "hbm", "achieved": round(ach, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+               "frac": round(ach / PEAK_HBM_GBS, 5), "traffic": None,
+               "kernel": "k_update_z_* (fused UpdateZ + counts + lnP)",
+               "algorithmic_bytes_per_launch": zbytes, "avg_launch_ms": round(z_avg * 1e3, 4),
+               "launches_timed": zn}
+
+
+def workload_c3(args):
+    """Large single chain: N=20000, L=500000 SNPs (2.5 GB of 2-bit genotypes),
+    K=8, F model; Z never materialised."""
+    import structure_amd as sa
+    N, L, K = 20000, 500000, 8
+    t0 = time.perf_counter()
+    d = sa.synth_snp(N, L, Ktrue=8, flo=0.005, fhi=0.05, qalpha=0.2, miss=0.01, seed=args.seed)
+    print(f"bench: synthetic data N={N} L={L} in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    warm, steps = args.warmup, args.steps
+    model = sa.make_model(N, L, K, 2, freqscorr=1, inferalpha=1, inferlambda=1, metrofreq=10,
+                          burnin=warm, numreps=steps, seed=args.seed, chain=0)
+    ch = sa.Chain(model, sa.GENO_SNP2, d["words"], d["nall"])
+    dt = _timed_run(ch, warm, steps)
+    zb = z_bytes_per_launch(N, L, K, 2)
+    timing, roof = _z_roofline(ch, warm + steps, min(steps, 5), zb)
+    ch.close()
+    sps = steps / dt
+    res = {"metric": "sweeps/s at N=20k,L=500k,K=8 (%HBM peak)", "value": round(sps, 3),
+           "unit": "sweeps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+           "ms_per_step": round(1e3 * dt / steps, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": f"synthetic (Balding-Nichols SNP, K_true=8, seed {args.seed})",
+           "config": {"workload": "config 3: N=20000 L=500000 K=8 biallelic 2-bit, F model + "
+                                  "alpha + lambda, METROFREQ=10, post-burn-in sweeps",
+                      "N": N, "L": L, "K": K, "genotype_format": "SNP2"},
+           "roofline": roof, "site_updates_per_s": round(N * L * K * sps, 1),
+           "phase_ms_per_sweep": {p: round(v[0] / max(min(steps, 5), 1), 4)
+                                  for p, v in timing.items()}}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(N, L, K, 500, 3, args.seed, ktrue=8, flo=0.005,
+                                           fhi=0.05, qalpha=0.2)
+    return res
+
+
+def _ragged_datasets(seed, n=64):
+    """Config 5: n independent data sets, N ~ U[100,5000], L log-uniform in
+    [1e3, 2e5], half SNP (Balding-Nichols) and half microsatellite (2-20
+    alleles), K_true and run K in [2, 8], Q ~ Dir(U[0.1,1]), missing U[0,5%]."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    import structure_amd as sa
+    rng = np.random.default_rng(seed)
+    specs = []
+    for j in range(n):
+        specs.append(dict(N=int(rng.integers(100, 5001)),
+                          L=int(round(10 ** rng.uniform(3, np.log10(2e5)))),
+                          snp=j % 2 == 0, ktrue=int(rng.integers(2, 9)), K=int(rng.integers(2, 9)),
+                          qalpha=float(rng.uniform(0.1, 1.0)), miss=float(rng.uniform(0, 0.05)),
+                          seed=int(rng.integers(1, 2 ** 31))))
+
+    def make(sp):
+        if sp["snp"]:
+            d = sa.synth_snp(sp["N"], sp["L"], Ktrue=sp["ktrue"], qalpha=sp["qalpha"],
+                             miss=sp["miss"], seed=sp["seed"], threads=16)
+            return sa.GENO_SNP2, d["words"], d["nall"], 2
+        d = sa.synth_microsat(sp["N"], sp["L"], amin=2, amax=20, Ktrue=sp["ktrue"],
+                              qalpha=sp["qalpha"], miss=sp["miss"], seed=sp["seed"])
+        fmt, words = sa.pack(d["geno"], sa.GENO_BYTE)
+        return fmt, words, d["nall"], d["A"]
+
+    # the generators are multithreaded themselves; two at a time overlaps packing
+    with cf.ThreadPoolExecutor(max_workers=2) as ex:
+        data = list(ex.map(make, specs))
+    return specs, data
+
+
+def _batch_vs_plain(models, datasets, doc, warm, sweeps, plain_sweeps=None):
+    """Chain-sweeps/s of all chains in one batch against the same chains each
+    run alone (one after the other) on the GPU."""
+    import structure_amd as sa
+    b = sa.Batch(models, datasets, doc)
+    t_batch = _timed_run(b, warm, sweeps)
+    b.close()
+    ps = plain_sweeps or sweeps
+    t_plain = 0.0
+    for m, j in zip(models, doc):
+        fmt, words, nall = datasets[j]
+        ch = sa.Chain(m, fmt, words, nall)
+        t_plain += _timed_run(ch, warm, ps)
+        ch.close()
+    n = len(models)
+    return n * sweeps / t_batch, n * ps / t_plain
+
+
+def workload_c5(args):
+    """64 ragged data sets, one chain each, all in one batch."""
+    import structure_amd as sa
+    t0 = time.perf_counter()
+    specs, data = _ragged_datasets(args.seed)
+    print(f"bench: 64 ragged data sets in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    warm, sweeps = args.warmup, args.steps
+    models = [sa.make_model(sp["N"], sp["L"], sp["K"], A, freqscorr=1, inferalpha=1,
+                            metrofreq=10, burnin=warm, numreps=sweeps, seed=args.seed + j,
+                            chain=j)
+              for j, (sp, (_, _, _, A)) in enumerate(zip(specs, data))]
+    dsets = [(f, w, na) for f, w, na, _ in data]
+    batched, plain = _batch_vs_plain(models, dsets, list(range(len(models))), warm, sweeps)
+    sites = sum(sp["N"] * sp["L"] * sp["K"] for sp in specs)
+    return {"metric": "chain-sweeps/s, 64 ragged data sets in one batch",
+            "value": round(batched, 3), "unit": "chain-sweeps/s", "n_gpus": 1, "steps": sweeps,
+            "warmup": warm, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": f"synthetic ragged (seed {args.seed})",
+            "config": {"workload": "config 5: 64 data sets, N~U[100,5000], L log-uniform "
+                                   "[1e3,2e5], half 2-bit SNP / half byte microsatellite "
+                                   "(2-20 alleles), K in [2,8], F model + alpha",
+                       "datasets": len(specs), "total_sites": sum(sp["N"] * sp["L"] for sp in specs)},
+            "plain_chain_sweeps_per_s": round(plain, 3), "speedup": round(batched / plain, 3),
+            "site_updates_per_s": round(sites * batched / len(specs), 1)}
+
+
+def workload_c1batch(args):
+    """Config 1 replicated: 100 chains of N=200, L=500 microsatellites, K=3,
+    no F model, alpha inferred; batched against one at a time."""
+    import structure_amd as sa
+    d = sa.synth_microsat(200, 500, amin=2, amax=10, Ktrue=3, qalpha=0.5, miss=0.02,
+                          seed=args.seed)
+    fmt, words = sa.pack(d["geno"])
+    A = d["A"]
+    warm, sweeps = args.warmup, args.steps
+    models = [sa.make_model(200, 500, 3, A, freqscorr=0, inferalpha=1, lambda0=1.0,
+                            metrofreq=10, burnin=warm, numreps=sweeps, seed=args.seed + r,
+                            chain=3) for r in range(100)]
+    batched, plain = _batch_vs_plain(models, [(fmt, words, d["nall"])], [0] * 100, warm,
+                                     sweeps, plain_sweeps=max(sweeps // 10, 20))
+    res = {"metric": "chain-sweeps/s, 100 replicate chains of config 1 in one batch",
+           "value": round(batched, 3), "unit": "chain-sweeps/s", "n_gpus": 1, "steps": sweeps,
+           "warmup": warm, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": f"synthetic microsatellites (seed {args.seed})",
+           "config": {"workload": "config 1 x 100 seeds: N=200 L=500 A in [2,10] K=3, "
+                                  "no F model, alpha inferred", "chains": 100},
+           "plain_chain_sweeps_per_s": round(plain, 3), "speedup": round(batched / plain, 3)}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_c1(d, A, args.seed)
+    return res
+
+
+def cpu_baseline_c1(d, A, seed):
+    """The oracle on one config-1 chain (one core), full-size data."""
+    import oracle
+    m = oracle.make_model(200, 500, 3, A, freqscorr=0, inferalpha=1, lambda0=1.0, metrofreq=10,
+                          burnin=0, seed=seed, chain=3)
+    ch = oracle.Chain(m, d["geno"], d["nall"])
+    ch.init()
+    ch.sweep(0)
+    n = 200
+    t0 = time.perf_counter()
+    for r in range(1, 1 + n):
+        ch.sweep(r)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "chain-sweeps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (FP64 C restatement) on one config-1 chain, {n} sweeps in "
+                      f"{dt:.1f} s (one chain per core: multiply by the cores used)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,11 +305,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-sweeps", type=int, default=10)
-    args = ap.parse_args()
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5", "c1batch"],
+                    help="c2 (default, the headline line), c3, c5 or c1batch")
+    args, _ = ap.parse_known_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "c2":
+        defaults = {"c3": (3, 10), "c5": (10, 20), "c1batch": (20, 200)}
+        w, k = defaults[args.workload]
+        if "--warmup" not in sys.argv:
+            args.warmup = w
+        if "--steps" not in sys.argv:
+            args.steps = k
+        fn = {"c3": workload_c3, "c5": workload_c5, "c1batch": workload_c1batch}
+        print(json.dumps(fn[args.workload](args)), flush=True)
+        return
 
     import structure_amd as sa
     N, L, K = args.N, args.L, args.K

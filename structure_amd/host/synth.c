@@ -289,6 +289,142 @@ int sgpu_synth_snp(int32_t N, int32_t L, int32_t Ktrue, double flo, double fhi, 
 
 /* ---- microsatellite panel ------------------------------------------------ */
 
+typedef struct {
+    int32_t i0, i1, L, Ktrue, amax;
+    double qalpha, miss;
+    uint64_t seed;
+    const int32_t *na;
+    const double *cf;
+    int8_t *geno;
+    double *qtrue;
+} ms_job;
+
+typedef struct {
+    int32_t N, L, l0, l1;
+    int8_t *geno;
+    int32_t *nall;
+    int failed;
+} ms_recode_job;
+
+static void *ms_recode_worker(void *arg)
+{
+    ms_recode_job *j = (ms_recode_job *)arg;
+    const int32_t n = j->l1 - j->l0;
+    if (n <= 0) {
+        return NULL;
+    }
+    int8_t *map = (int8_t *)malloc((size_t)n * 128);
+    if (!map) {
+        j->failed = 1;
+        return NULL;
+    }
+    memset(map, -1, (size_t)n * 128);
+    int32_t *nall = j->nall + j->l0;
+    memset(nall, 0, sizeof(int32_t) * (size_t)n);
+    for (int32_t i = 0; i < j->N; i++) {
+        for (int c = 0; c < 2; c++) {
+            int8_t *row = j->geno + ((size_t)i * 2 + c) * j->L + j->l0;
+            for (int32_t l = 0; l < n; l++) {
+                const int8_t x = row[l];
+                if (x == -9) {
+                    continue;
+                }
+                int8_t *m = map + (size_t)l * 128 + x;
+                if (*m < 0) {
+                    *m = (int8_t)nall[l]++;
+                }
+                row[l] = *m;
+            }
+        }
+    }
+    free(map);
+    return NULL;
+}
+
+/* Individuals i0..i1-1 of the microsatellite panel; each has its own stream,
+ * so the split over threads does not change the data. */
+static void *ms_worker(void *arg)
+{
+    const ms_job *j = (const ms_job *)arg;
+    const int32_t L = j->L, Ktrue = j->Ktrue, amax = j->amax;
+    const int32_t *na = j->na;
+    const double *cf = j->cf;
+    const double miss = j->miss;
+    const uint64_t seed = j->seed;
+    int8_t *geno = j->geno;
+    double *qtrue = j->qtrue;
+    const int32_t ni = j->i1 - j->i0;
+    if (ni <= 0) {
+        return NULL;
+    }
+    /* every individual keeps its own stream; loci are visited in blocks whose
+     * frequency table fits in cache, each individual still in locus order */
+    xrng *st = (xrng *)malloc(sizeof(xrng) * (size_t)ni);
+    double *cqs = (double *)malloc(sizeof(double) * (size_t)ni * Ktrue);
+    double *sums = (double *)malloc(sizeof(double) * (size_t)ni);
+    if (!st || !cqs || !sums) {
+        free(st);
+        free(cqs);
+        free(sums);
+        return (void *)1;
+    }
+    double qa[64], q[64];
+    for (int k = 0; k < Ktrue; k++) {
+        qa[k] = j->qalpha;
+    }
+    for (int32_t t = 0; t < ni; t++) {
+        const int32_t i = j->i0 + t;
+        xrng_seed(&st[t], seed, (uint64_t)i + 1);
+        xdirichlet(&st[t], qa, Ktrue, q);
+        if (qtrue) {
+            memcpy(qtrue + (size_t)i * Ktrue, q, sizeof(double) * Ktrue);
+        }
+        double s = 0.0;
+        for (int k = 0; k < Ktrue; k++) {
+            s += q[k];
+            cqs[(size_t)t * Ktrue + k] = s;
+        }
+        sums[t] = s;
+    }
+    const int32_t LB = 1024;
+    for (int32_t lb = 0; lb < L; lb += LB) {
+        const int32_t le = lb + LB < L ? lb + LB : L;
+        for (int32_t t = 0; t < ni; t++) {
+            const int32_t i = j->i0 + t;
+            xrng *ri = &st[t];
+            const double *cq = cqs + (size_t)t * Ktrue;
+            const double s = sums[t];
+            int8_t *c0 = geno + (size_t)i * 2 * L;
+            int8_t *c1 = c0 + L;
+            for (int32_t l = lb; l < le; l++) {
+                if (xunif(ri) < miss) {
+                    c0[l] = c1[l] = -9;
+                    continue;
+                }
+                for (int c = 0; c < 2; c++) {
+                    const double u = xunif(ri) * s;
+                    int z = 0;
+                    while (z < Ktrue - 1 && u >= cq[z]) {
+                        z++;
+                    }
+                    const double *cz = cf + ((size_t)l * Ktrue + z) * amax;
+                    const double v = xunif(ri) * cz[na[l] - 1];
+                    int a = 0;
+                    while (a < na[l] - 1 && v >= cz[a]) {
+                        a++;
+                    }
+                    (c ? c1 : c0)[l] = (int8_t)a;
+                }
+            }
+        }
+    }
+    free(st);
+    free(cqs);
+    free(sums);
+    return NULL;
+}
+
+
 int sgpu_synth_microsat(int32_t N, int32_t L, int32_t amin, int32_t amax, int32_t Ktrue,
                         double qalpha, double miss, uint64_t seed, int8_t *geno, int32_t *nall,
                         double *qtrue)
@@ -321,67 +457,53 @@ int sgpu_synth_microsat(int32_t N, int32_t L, int32_t amin, int32_t amax, int32_
             }
         }
     }
-    double qa[64], q[64], cq[64];
-    for (int k = 0; k < Ktrue; k++) {
-        qa[k] = qalpha;
+    int nt = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    nt = nt < 1 ? 1 : (nt > 16 ? 16 : nt);
+    if (nt > N) {
+        nt = N;
     }
-    for (int32_t i = 0; i < N; i++) {
-        xrng ri;
-        xrng_seed(&ri, seed, (uint64_t)i + 1);
-        xdirichlet(&ri, qa, Ktrue, q);
-        if (qtrue) {
-            memcpy(qtrue + (size_t)i * Ktrue, q, sizeof(double) * Ktrue);
-        }
-        double s = 0.0;
-        for (int k = 0; k < Ktrue; k++) {
-            s += q[k];
-            cq[k] = s;
-        }
-        int8_t *c0 = geno + (size_t)i * 2 * L;
-        int8_t *c1 = c0 + L;
-        for (int32_t l = 0; l < L; l++) {
-            if (xunif(&ri) < miss) {
-                c0[l] = c1[l] = -9;
-                continue;
-            }
-            for (int c = 0; c < 2; c++) {
-                const double u = xunif(&ri) * s;
-                int z = 0;
-                while (z < Ktrue - 1 && u >= cq[z]) {
-                    z++;
-                }
-                const double *cz = cf + ((size_t)l * Ktrue + z) * amax;
-                const double v = xunif(&ri) * cz[na[l] - 1];
-                int a = 0;
-                while (a < na[l] - 1 && v >= cz[a]) {
-                    a++;
-                }
-                (c ? c1 : c0)[l] = (int8_t)a;
-            }
-        }
+    pthread_t th[16];
+    ms_job jobs[16];
+    for (int t = 0; t < nt; t++) {
+        ms_job jb = {(int32_t)((int64_t)N * t / nt), (int32_t)((int64_t)N * (t + 1) / nt), L,
+                     Ktrue, amax, qalpha, miss, seed, na, cf, geno, qtrue};
+        jobs[t] = jb;
+        pthread_create(&th[t], NULL, ms_worker, &jobs[t]);
     }
-    /* first-appearance recoding over (ind, line) (datain.c:846-895) */
+    int gen_failed = 0;
+    for (int t = 0; t < nt; t++) {
+        void *ret = NULL;
+        pthread_join(th[t], &ret);
+        gen_failed |= ret != NULL;
+    }
+    if (gen_failed) {
+        free(na);
+        free(cf);
+        return SGPU_ERR_NOMEM;
+    }
+    /* first-appearance recoding over (ind, line) (datain.c:846-895): each
+     * thread owns a range of loci and streams every row over it, with one
+     * code table per locus */
+    ms_recode_job rj[16];
+    for (int t = 0; t < nt; t++) {
+        ms_recode_job jb = {N, L, (int32_t)((int64_t)L * t / nt),
+                            (int32_t)((int64_t)L * (t + 1) / nt), geno, nall, 0};
+        rj[t] = jb;
+        pthread_create(&th[t], NULL, ms_recode_worker, &rj[t]);
+    }
+    int failed = 0;
+    for (int t = 0; t < nt; t++) {
+        pthread_join(th[t], NULL);
+        failed |= rj[t].failed;
+    }
+    if (failed) {
+        free(na);
+        free(cf);
+        return SGPU_ERR_NOMEM;
+    }
     int maxk = 0;
-    int8_t map[128];
     for (int32_t l = 0; l < L; l++) {
-        memset(map, -1, sizeof(map));
-        int k = 0;
-        for (int32_t i = 0; i < N; i++) {
-            for (int c = 0; c < 2; c++) {
-                int8_t *x = geno + ((size_t)i * 2 + c) * L + l;
-                if (*x == -9) {
-                    continue;
-                }
-                if (map[(int)*x] < 0) {
-                    map[(int)*x] = (int8_t)k++;
-                }
-                *x = map[(int)*x];
-            }
-        }
-        nall[l] = k;
-        if (k > maxk) {
-            maxk = k;
-        }
+        maxk = nall[l] > maxk ? nall[l] : maxk;
     }
     free(na);
     free(cf);
