@@ -345,7 +345,8 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__res
     }
 }
 
-// Byte-per-copy path (any allele count, runtime K).
+// Byte-per-copy path (any allele count), K templated.
+template <int K>
 __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__restrict__ cds,
                                                            const BlockRef *__restrict__ map,
                                                            const uint32_t *__restrict__ repp)
@@ -363,11 +364,15 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__res
     if (i >= m.N) {
         return;
     }
-    const int K = m.K;
-    double tq[kMaxK], cq[kMaxK];
-    dirichlet_q(m, alpha, nullptr, rep, i, PH_QMETRO, tq);
-    for (int k = 0; k < K; k++) {
-        cq[k] = Q[(size_t)i * K + k];
+    double tq[K], cq[K];
+    {
+        double tmp[kMaxK];
+        dirichlet_q(m, alpha, nullptr, rep, i, PH_QMETRO, tmp);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            tq[k] = tmp[k];
+            cq[k] = Q[(size_t)i * K + k];
+        }
     }
     const int l0 = by * loci_per_range;
     const int l1 = min(l0 + loci_per_range, m.L);
@@ -380,6 +385,7 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__res
         const SGPU_G double *Pl = P + (size_t)l * K * m.A;
         if (xa != 0xff) {
             double tp = 0.0, tm = 0.0;
+#pragma unroll
             for (int k = 0; k < K; k++) {
                 tp = fma(tq[k], Pl[k * m.A + xa], tp);
                 tm = fma(cq[k], Pl[k * m.A + xa], tm);
@@ -389,6 +395,7 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__res
         }
         if (xb != 0xff) {
             double tp = 0.0, tm = 0.0;
+#pragma unroll
             for (int k = 0; k < K; k++) {
                 tp = fma(tq[k], Pl[k * m.A + xb], tp);
                 tm = fma(cq[k], Pl[k * m.A + xb], tm);

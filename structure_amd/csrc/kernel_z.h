@@ -23,21 +23,6 @@
 
 namespace sgpu {
 
-// Per-individual count accumulation: mask of lanes whose copy went to pop k.
-__device__ __forceinline__ void add_ind_counts(uint32_t *s_cnt, int slot, int K, int lane,
-                                               const int z0, const int z1, const bool v0,
-                                               const bool v1)
-{
-    for (int k = 0; k < K; k++) {
-        const unsigned long long m0 = __ballot(v0 && z0 == k);
-        const unsigned long long m1 = __ballot(v1 && z1 == k);
-        const uint32_t n = (uint32_t)(__popcll(m0) + __popcll(m1));
-        if (lane == 0 && n) {
-            atomicAdd(&s_cnt[slot * K + k], n);
-        }
-    }
-}
-
 #ifndef SGPU_ZVAR
 #define SGPU_ZVAR 1
 #endif
@@ -314,135 +299,189 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
 }
 
 // ---------------------------------------------------------------------------
-// Byte-per-copy path (any allele count): runtime K <= kMaxK.  A word holds the
-// two copies of two individuals, so one Philox block serves one word.  The
-// lane owns its locus's partial histogram row and updates it in place.
+// Byte-per-copy path (any allele count), K templated.  One lane per genotype
+// word row, i.e. per pair of individuals (2h, 2h+1), 256 rows per block; the
+// block walks a tile of loci.  Per locus the K x A allele frequencies are
+// wave-uniform, so they are staged in LDS a chunk of loci at a time, and the
+// locus's allele-by-population histogram is a set of LDS bins every lane adds
+// into.  Per-individual counts stay in lane registers as "z > k" counts, as in
+// the 2-bit path.  One Philox block per (locus, pair): words 0,1 serve the two
+// copies of individual 2h, words 2,3 those of 2h+1, the j-th non-missing copy
+// taking the j-th word of its individual (randGen.c:23-36).
 // ---------------------------------------------------------------------------
-template <bool INIT, bool LIKE>
-__global__ __launch_bounds__(kZLoci) void k_update_z_byte(const ChainDesc *__restrict__ cds,
-                                                          const BlockRef *__restrict__ map,
-                                                          const uint32_t *__restrict__ repp)
+constexpr int kByteRows = 256;   // word rows (individual pairs) per UpdateZ block
+
+template <int K, bool INIT, bool LIKE>
+__device__ __forceinline__ void byte_copy(int x, bool valid, uint32_t w, const double *q,
+                                          const double *s_pl, int A, uint32_t *s_hl,
+                                          uint32_t *T, uint32_t &V, LikeAcc &acc, int &zout)
+{
+    if (!valid) {
+        return;
+    }
+    V += 1;
+    int z = 0;
+    if (INIT) {
+        const uint32_t u = (uint32_t)(((uint64_t)w * (uint32_t)K) >> 32);
+#pragma unroll
+        for (int k = 0; k < K - 1; k++) {
+            const bool gt = u > (uint32_t)k;
+            T[k] += gt;
+            z += gt;
+        }
+    } else {
+        double pre[K];
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            s = s + q[k] * s_pl[k * A + x];   // product and running sum in the reference's order
+            pre[k] = s;
+        }
+        if (LIKE) {
+            acc.mul(s);
+        }
+        const double r = (double)w * (s * 0x1p-32);   // u * S with u = w 2^-32, exactly
+#pragma unroll
+        for (int k = 0; k < K - 1; k++) {
+            const bool gt = pre[k] < r;   // z > k
+            T[k] += gt;
+            z += gt;
+        }
+    }
+    atomicAdd(&s_hl[z * A + x], 1u);
+    zout = z;
+}
+
+template <int K, bool INIT, bool LIKE, bool ZOUT>
+__global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__restrict__ cds,
+                                                             const BlockRef *__restrict__ map,
+                                                             const uint32_t *__restrict__ repp)
 {
     const BlockRef br = map[blockIdx.x];
     const ChainDesc &C = cds[br.chain];
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const int bx = (int)br.bx, by = (int)br.by;
-    struct {
-        const uint32_t *geno;
-        const double *Q, *P;
-        int32_t *nApart, *nQpart;
-        double *likepart;
-        int8_t *zout;
-        int N, L, A, Lpad, G, rows_per_tile, n_lt;
-        uint32_t seed, chain;
-    } a = {C.geno, C.Q, C.P, C.nApart, C.nQpart, C.likepart, C.zout, C.m.N, C.m.L, C.m.A,
-           C.Lpad, C.G, C.rows_per_tile, C.n_lt, C.m.seed, C.m.chain};
-    extern __shared__ uint32_t s_cnt[];
-    const int lane = threadIdx.x & 63;
-    const int K = C.m.K;
-    const int l = bx * kZLoci + threadIdx.x;
-    const bool lok = l < a.L;
-    const int row0 = by * a.rows_per_tile;
-    const int row1 = min(row0 + a.rows_per_tile, a.G);
-    const int ibase = row0 * 2;
-    const int ntile = min((row1 - row0) * 2, a.N - ibase);
+    const int N = C.m.N, L = C.m.L, A = C.m.A, Lpad = C.Lpad, G = C.G;
+    const int zt = C.zt_loci, chunk = C.z_chunk, n_lt = C.n_lt;
+    const uint32_t seed = C.m.seed, chain = C.m.chain;
+    const SGPU_G uint32_t *geno = g(C.geno);
+    const SGPU_G double *Q = g(C.Q);
+    const SGPU_G double *P = g(C.P);
+    SGPU_G int32_t *nApart = g(C.nApart);
+    SGPU_G int32_t *nQpart = g(C.nQpart);
+    SGPU_G double *likepart = g(C.likepart);
+    SGPU_G int8_t *zout = g(C.zout);
+    extern __shared__ double s_dyn[];
+    double *s_p = s_dyn;                                  // [chunk][K][A] frequencies
+    uint32_t *s_h = (uint32_t *)(s_dyn + (size_t)chunk * K * A);   // [chunk][K][A] counts
 
-    for (int t = threadIdx.x; t < ntile * K; t += kZLoci) {
-        s_cnt[t] = 0;
+    const int h = by * kByteRows + threadIdx.x;   // word row of this lane
+    const bool rowok = h < G;
+    const int i0 = 2 * h, i1 = 2 * h + 1;
+    const bool ok0 = rowok && i0 < N, ok1 = rowok && i1 < N;
+    double q0[K], q1[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        q0[k] = (!INIT && ok0) ? Q[(size_t)i0 * K + k] : 0.0;
+        q1[k] = (!INIT && ok1) ? Q[(size_t)i1 * K + k] : 0.0;
     }
-    int32_t *hist = a.nApart + ((size_t)by * a.L + (lok ? l : 0)) * K * a.A;
-    if (lok) {
-        for (int t = 0; t < K * a.A; t++) {
-            hist[t] = 0;
-        }
+    uint32_t T0[K], T1[K];   // entries 0..K-2: copies with z > k
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        T0[k] = 0;
+        T1[k] = 0;
     }
-    __syncthreads();
-    const double *Pl = a.P + (size_t)(lok ? l : 0) * K * a.A;
+    uint32_t V0 = 0, V1 = 0;
     LikeAcc acc;
-
-    for (int g = row0; g < row1; g++) {
-        const uint32_t word = a.geno[(size_t)g * a.Lpad + l];
-        const int i0 = g * 2;
-        const u32x4 w = philox4x32_10(rep, (uint32_t)l, (uint32_t)(i0 >> 1),
-                                      (INIT ? PH_INITZ : PH_Z) << 24, a.seed, a.chain);
-        for (int h = 0; h < 2; h++) {
-            const int i = i0 + h;
-            if (i >= a.N) {
-                break;
+    const int lt0 = bx * zt, lt1 = min(lt0 + zt, L);
+    const int KA = K * A;
+    for (int c0 = lt0; c0 < lt1; c0 += chunk) {
+        const int nc = min(chunk, lt1 - c0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < nc * KA; t += kByteRows) {
+            if (!INIT) {
+                s_p[t] = P[(size_t)c0 * KA + t];
             }
-            const int x0 = (int)((word >> (16 * h)) & 0xffu);
-            const int x1 = (int)((word >> (16 * h + 8)) & 0xffu);
-            const bool v0 = lok && x0 != 0xff;
-            const bool v1 = lok && x1 != 0xff;
-            // uniforms go to the non-missing copies in order (randGen.c:23-36)
-            const uint32_t wa = h ? w.z : w.x;
-            const uint32_t wb1 = v0 ? (h ? w.w : w.y) : wa;
-            int z0 = 0, z1 = 0;
-            if (INIT) {
-                z0 = (int)(((uint64_t)wa * (uint32_t)K) >> 32);
-                z1 = (int)(((uint64_t)wb1 * (uint32_t)K) >> 32);
-            } else {
-                double q[kMaxK];
-                for (int k = 0; k < K; k++) {
-                    q[k] = a.Q[(size_t)i * K + k];
+            s_h[t] = 0u;
+        }
+        __syncthreads();
+        for (int j = 0; j < nc; j++) {
+            const int l = c0 + j;
+            const uint32_t word = rowok ? geno[(size_t)h * Lpad + l] : 0xffffffffu;
+            if (word != 0xffffffffu) {
+                const u32x4 w = philox4x32_10(rep, (uint32_t)l, (uint32_t)h,
+                                              (INIT ? PH_INITZ : PH_Z) << 24, seed, chain);
+                const double *s_pl = s_p + (size_t)j * KA;
+                uint32_t *s_hl = s_h + (size_t)j * KA;
+                const int xa = (int)(word & 0xffu), xb = (int)((word >> 8) & 0xffu);
+                const int xc = (int)((word >> 16) & 0xffu), xd = (int)((word >> 24) & 0xffu);
+                const bool va = ok0 && xa != 0xff, vb = ok0 && xb != 0xff;
+                const bool vc = ok1 && xc != 0xff, vd = ok1 && xd != 0xff;
+                int za = -9, zb = -9, zc = -9, zd = -9;
+                byte_copy<K, INIT, LIKE>(xa, va, w.x, q0, s_pl, A, s_hl, T0, V0, acc, za);
+                byte_copy<K, INIT, LIKE>(xb, vb, va ? w.y : w.x, q0, s_pl, A, s_hl, T0, V0, acc,
+                                         zb);
+                byte_copy<K, INIT, LIKE>(xc, vc, w.z, q1, s_pl, A, s_hl, T1, V1, acc, zc);
+                byte_copy<K, INIT, LIKE>(xd, vd, vc ? w.w : w.z, q1, s_pl, A, s_hl, T1, V1, acc,
+                                         zd);
+                if (LIKE) {
+                    acc.renorm();
                 }
-                if (v0) {
-                    double s = 0.0;
-                    for (int k = 0; k < K; k++) {
-                        s = s + q[k] * Pl[k * a.A + x0];
+                if (ZOUT) {
+                    if (ok0) {
+                        zout[((size_t)i0 * 2 + 0) * L + l] = (int8_t)za;
+                        zout[((size_t)i0 * 2 + 1) * L + l] = (int8_t)zb;
                     }
-                    const double r = uz(wa) * s;
-                    double run = 0.0;
-                    for (int k = 0; k < K - 1; k++) {
-                        run = run + q[k] * Pl[k * a.A + x0];
-                        z0 += run < r;
-                    }
-                    if (LIKE) {
-                        acc.mul(s);
-                    }
-                }
-                if (v1) {
-                    double s = 0.0;
-                    for (int k = 0; k < K; k++) {
-                        s = s + q[k] * Pl[k * a.A + x1];
-                    }
-                    const double r = uz(wb1) * s;
-                    double run = 0.0;
-                    for (int k = 0; k < K - 1; k++) {
-                        run = run + q[k] * Pl[k * a.A + x1];
-                        z1 += run < r;
-                    }
-                    if (LIKE) {
-                        acc.mul(s);
+                    if (ok1) {
+                        zout[((size_t)i1 * 2 + 0) * L + l] = (int8_t)zc;
+                        zout[((size_t)i1 * 2 + 1) * L + l] = (int8_t)zd;
                     }
                 }
-            }
-            if (v0) {
-                hist[z0 * a.A + x0] += 1;
-            }
-            if (v1) {
-                hist[z1 * a.A + x1] += 1;
-            }
-            add_ind_counts(s_cnt, i - ibase, K, lane, z0, z1, v0, v1);
-            if (a.zout && lok) {
-                a.zout[((size_t)i * 2 + 0) * a.L + l] = v0 ? (int8_t)z0 : (int8_t)-9;
-                a.zout[((size_t)i * 2 + 1) * a.L + l] = v1 ? (int8_t)z1 : (int8_t)-9;
+            } else if (ZOUT && rowok) {
+                if (ok0) {
+                    zout[((size_t)i0 * 2 + 0) * L + l] = -9;
+                    zout[((size_t)i0 * 2 + 1) * L + l] = -9;
+                }
+                if (ok1) {
+                    zout[((size_t)i1 * 2 + 0) * L + l] = -9;
+                    zout[((size_t)i1 * 2 + 1) * L + l] = -9;
+                }
             }
         }
-        if (LIKE) {
-            acc.renorm();
+        __syncthreads();
+        // this block alone covers (individual tile by, loci c0..c0+nc): plain stores
+        SGPU_G int32_t *dst = nApart + ((size_t)by * L + c0) * KA;
+        for (int t = threadIdx.x; t < nc * KA; t += kByteRows) {
+            dst[t] = (int32_t)s_h[t];
         }
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < ntile * K; t += kZLoci) {
-        a.nQpart[((size_t)bx * a.N + ibase) * K + t] = (int32_t)s_cnt[t];
+    // counts per population from the "z > k" counts
+    if (ok0) {
+        SGPU_G int32_t *d0 = nQpart + ((size_t)bx * N + i0) * K;
+        uint32_t prev = V0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t tk = (k < K - 1) ? T0[k] : 0u;
+            d0[k] = (int32_t)(prev - tk);
+            prev = tk;
+        }
+    }
+    if (ok1) {
+        SGPU_G int32_t *d1 = nQpart + ((size_t)bx * N + i1) * K;
+        uint32_t prev = V1;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t tk = (k < K - 1) ? T1[k] : 0u;
+            d1[k] = (int32_t)(prev - tk);
+            prev = tk;
+        }
     }
     if (LIKE) {
-        __shared__ double s_red[kZLoci / 64];
-        const double v = lok ? acc.log_value() : 0.0;
-        const double tot = block_sum<kZLoci>(v, s_red);
+        __shared__ double s_red[kByteRows / 64];
+        const double v = (ok0 || ok1) ? acc.log_value() : 0.0;
+        const double tot = block_sum<kByteRows>(v, s_red);
         if (threadIdx.x == 0) {
-            a.likepart[(size_t)by * a.n_lt + bx] = tot;
+            likepart[(size_t)by * n_lt + bx] = tot;
         }
     }
 }

@@ -61,6 +61,8 @@ constexpr int kZBlocksDefault = 2048;
 constexpr int kMetroBlocksDefault = 2048;
 // LDS per UpdateZ block for the per-individual counts (keeps 3 blocks per CU).
 constexpr size_t kZLdsMax = 48 * 1024;
+// LDS per byte-format UpdateZ block for the staged frequencies and counts.
+constexpr size_t kByteLdsMax = 60 * 1024;
 
 // One launch over a set of chains: the block map and its LDS need.
 struct Plan {
@@ -208,8 +210,17 @@ auto all_chains = [](const HostChain &) { return true; };
 typedef void (*zkern_t)(const ChainDesc *, const BlockRef *, const uint32_t *);
 
 template <int K>
-zkern_t pick_snp2(int mode)
+zkern_t pick_z(int fmt, int mode)
 {
+    if (fmt == SGPU_GENO_BYTE) {
+        switch (mode) {
+        case 0: return k_update_z_byte<K, true, false, false>;
+        case 1: return k_update_z_byte<K, false, false, false>;
+        case 2: return k_update_z_byte<K, false, true, false>;
+        case 3: return k_update_z_byte<K, false, false, true>;
+        default: return k_update_z_byte<K, false, true, true>;
+        }
+    }
     switch (mode) {
     case 0: return k_update_z_snp2<K, true, false, false>;
     case 1: return k_update_z_snp2<K, false, false, false>;
@@ -222,58 +233,54 @@ zkern_t pick_snp2(int mode)
 // mode 0 = initial Z, 1 = update, 2 = update + lnP terms (+2 with a debug copy of Z)
 zkern_t z_kernel(int fmt, int K, int mode)
 {
-    if (fmt == SGPU_GENO_BYTE) {
-        switch (mode) {
-        case 0: return k_update_z_byte<true, false>;
-        case 1: case 3: return k_update_z_byte<false, false>;
-        default: return k_update_z_byte<false, true>;
-        }
-    }
     switch (K) {
-    case 1: return pick_snp2<1>(mode);
-    case 2: return pick_snp2<2>(mode);
-    case 3: return pick_snp2<3>(mode);
-    case 4: return pick_snp2<4>(mode);
-    case 5: return pick_snp2<5>(mode);
-    case 6: return pick_snp2<6>(mode);
-    case 7: return pick_snp2<7>(mode);
-    case 8: return pick_snp2<8>(mode);
-    case 9: return pick_snp2<9>(mode);
-    case 10: return pick_snp2<10>(mode);
-    case 11: return pick_snp2<11>(mode);
-    case 12: return pick_snp2<12>(mode);
-    case 13: return pick_snp2<13>(mode);
-    case 14: return pick_snp2<14>(mode);
-    case 15: return pick_snp2<15>(mode);
-    case 16: return pick_snp2<16>(mode);
+    case 1: return pick_z<1>(fmt, mode);
+    case 2: return pick_z<2>(fmt, mode);
+    case 3: return pick_z<3>(fmt, mode);
+    case 4: return pick_z<4>(fmt, mode);
+    case 5: return pick_z<5>(fmt, mode);
+    case 6: return pick_z<6>(fmt, mode);
+    case 7: return pick_z<7>(fmt, mode);
+    case 8: return pick_z<8>(fmt, mode);
+    case 9: return pick_z<9>(fmt, mode);
+    case 10: return pick_z<10>(fmt, mode);
+    case 11: return pick_z<11>(fmt, mode);
+    case 12: return pick_z<12>(fmt, mode);
+    case 13: return pick_z<13>(fmt, mode);
+    case 14: return pick_z<14>(fmt, mode);
+    case 15: return pick_z<15>(fmt, mode);
+    case 16: return pick_z<16>(fmt, mode);
     default: return nullptr;
     }
 }
 
 typedef void (*mkern_t)(const ChainDesc *, const BlockRef *, const uint32_t *);
 
+template <int K>
+mkern_t pick_metro(int fmt)
+{
+    return fmt == SGPU_GENO_BYTE ? k_metro_ratio_byte<K> : k_metro_ratio_snp2<K>;
+}
+
 mkern_t metro_kernel(int fmt, int K)
 {
-    if (fmt == SGPU_GENO_BYTE) {
-        return k_metro_ratio_byte;
-    }
     switch (K) {
-    case 1: return k_metro_ratio_snp2<1>;
-    case 2: return k_metro_ratio_snp2<2>;
-    case 3: return k_metro_ratio_snp2<3>;
-    case 4: return k_metro_ratio_snp2<4>;
-    case 5: return k_metro_ratio_snp2<5>;
-    case 6: return k_metro_ratio_snp2<6>;
-    case 7: return k_metro_ratio_snp2<7>;
-    case 8: return k_metro_ratio_snp2<8>;
-    case 9: return k_metro_ratio_snp2<9>;
-    case 10: return k_metro_ratio_snp2<10>;
-    case 11: return k_metro_ratio_snp2<11>;
-    case 12: return k_metro_ratio_snp2<12>;
-    case 13: return k_metro_ratio_snp2<13>;
-    case 14: return k_metro_ratio_snp2<14>;
-    case 15: return k_metro_ratio_snp2<15>;
-    case 16: return k_metro_ratio_snp2<16>;
+    case 1: return pick_metro<1>(fmt);
+    case 2: return pick_metro<2>(fmt);
+    case 3: return pick_metro<3>(fmt);
+    case 4: return pick_metro<4>(fmt);
+    case 5: return pick_metro<5>(fmt);
+    case 6: return pick_metro<6>(fmt);
+    case 7: return pick_metro<7>(fmt);
+    case 8: return pick_metro<8>(fmt);
+    case 9: return pick_metro<9>(fmt);
+    case 10: return pick_metro<10>(fmt);
+    case 11: return pick_metro<11>(fmt);
+    case 12: return pick_metro<12>(fmt);
+    case 13: return pick_metro<13>(fmt);
+    case 14: return pick_metro<14>(fmt);
+    case 15: return pick_metro<15>(fmt);
+    case 16: return pick_metro<16>(fmt);
     default: return nullptr;
     }
 }
@@ -285,7 +292,8 @@ int launch_z(sgpu_ctx *c, int mode)
         if (!k) {
             return fail(SGPU_ERR_UNSUPPORTED, "no UpdateZ kernel for K=%d", g.K);
         }
-        hipLaunchKernelGGL(k, dim3(g.plan.nblocks), dim3(kZLoci), g.plan.lds, c->stream, c->cds,
+        const int threads = g.fmt == SGPU_GENO_BYTE ? kByteRows : kZLoci;
+        hipLaunchKernelGGL(k, dim3(g.plan.nblocks), dim3(threads), g.plan.lds, c->stream, c->cds,
                            g.plan.map, (const uint32_t *)c->repd);
         LAUNCH_OK();
     }
@@ -548,8 +556,36 @@ int upload_dataset(sgpu_ctx *c, Dataset &ds, const sgpu_data *d)
 
 // Tiles of the chain's UpdateZ and Metropolis launches for the batch's block
 // targets, then its device state.
-int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, int total_lt, int total_qb,
-                int z_target, int metro_target)
+// Tile sizes shared by every chain of a batch, so that blocks of one launch
+// carry about the same work whatever the chain: word rows per 2-bit UpdateZ
+// block and loci per Metropolis block, each set so that the whole batch comes
+// to about the target number of blocks.
+struct BatchTiles {
+    int snp_rows;   // 16-individual word rows per 2-bit UpdateZ block (before the LDS cap)
+    int metro_lpr;  // loci per Metropolis block (multiple of 4)
+};
+
+BatchTiles batch_tiles(const sgpu_model *models, const sgpu_data *data,
+                       const int32_t *data_of_chain, int nchains, int z_target,
+                       int metro_target)
+{
+    long long snp_work = 0, metro_work = 0;   // in (row x locus tile) and (256 inds x locus)
+    for (int ci = 0; ci < nchains; ci++) {
+        const sgpu_model &m = models[ci];
+        if (data[data_of_chain[ci]].format == SGPU_GENO_SNP2) {
+            snp_work += (long long)cdiv(m.N, 16) * (long long)cdiv(m.L, kZLoci);
+        }
+        metro_work += (long long)cdiv(m.N, kNT) * m.L;
+    }
+    BatchTiles t;
+    t.snp_rows = (int)std::max<long long>(1, snp_work / std::max(z_target, 1));
+    const long long lpr = std::max<long long>(4, (metro_work + metro_target - 1) /
+                                                     std::max(metro_target, 1));
+    t.metro_lpr = (int)std::min<long long>((lpr + 3) / 4 * 4, 1 << 30);
+    return t;
+}
+
+int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &bt)
 {
     const sgpu_model &m = h.m;
     ChainDesc &d = h.d;
@@ -563,21 +599,39 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, int total_lt, int 
     d.ahist = ds.ahist;
     d.nhist = ds.nhist;
     d.nloci_nonzero = ds.nloci_nonzero;
-    const int inds_per_row = ds.fmt == SGPU_GENO_SNP2 ? 16 : 2;
-    d.n_lt = (int)cdiv(L, kZLoci);
-    int n_it = (int)std::min<size_t>(std::max<size_t>(cdiv(z_target, total_lt), 1), d.G);
-    int rows = (int)cdiv(d.G, n_it);
-    while ((size_t)rows * inds_per_row * K * 4 > kZLdsMax && rows > 1) {
-        rows = (rows + 1) / 2;
+    if (ds.fmt == SGPU_GENO_SNP2) {
+        // one locus per lane, 256 loci per block; individual tiles sized so that
+        // one launch over the batch has about z_target blocks, LDS counts <= 48 KiB
+        d.zt_loci = kZLoci;
+        d.z_chunk = 0;
+        d.n_lt = (int)cdiv(L, kZLoci);
+        int rows = std::min(bt.snp_rows, d.G);
+        while ((size_t)rows * 16 * K * 4 > kZLdsMax && rows > 1) {
+            rows = (rows + 1) / 2;
+        }
+        d.n_it = (int)cdiv(d.G, rows);
+        d.rows_per_tile = (int)cdiv(d.G, d.n_it);   // even tiles
+        h.z_lds = (size_t)rows * 16 * K * sizeof(uint32_t);
+    } else {
+        // one individual pair per lane, 256 pairs per block; short data sets take
+        // 64-locus tiles so that small chains still spread over many blocks; a
+        // chunk of loci's K x A frequencies (FP64) and counts (u32) sit in LDS
+        d.zt_loci = L < 16384 ? 64 : kZLoci;
+        d.n_lt = (int)cdiv(L, d.zt_loci);
+        d.rows_per_tile = kByteRows;
+        d.n_it = (int)cdiv(d.G, kByteRows);
+        const size_t per_locus = (size_t)K * A * (sizeof(double) + sizeof(uint32_t));
+        d.z_chunk = (int)std::max<size_t>(1, std::min<size_t>(kByteLdsMax / per_locus,
+                                                              (size_t)d.zt_loci));
+        h.z_lds = (size_t)d.z_chunk * per_locus;
     }
-    d.rows_per_tile = rows;
-    d.n_it = (int)cdiv(d.G, rows);
-    h.z_lds = (size_t)rows * inds_per_row * K * sizeof(uint32_t);
     d.q_blocks = (int)cdiv(N, kNT);
     d.l_blocks = (int)cdiv(L, kNT);
-    int n_ranges = (int)std::max<size_t>(1, std::min<size_t>(cdiv(metro_target, total_qb), L));
-    d.loci_per_range = (int)(cdiv(cdiv(L, n_ranges), 4) * 4);   // multiple of 4 loci
-    d.n_ranges = (int)cdiv(L, d.loci_per_range);
+    {
+        const int n_ranges = (int)cdiv(L, std::min<size_t>(bt.metro_lpr, cdiv(L, 4) * 4));
+        d.loci_per_range = (int)(cdiv(cdiv(L, n_ranges), 4) * 4);   // even, multiple of 4 loci
+        d.n_ranges = (int)cdiv(L, d.loci_per_range);
+    }
     d.zout = nullptr;
 
     int rc;
@@ -804,11 +858,8 @@ int sgpu_create_batch(const sgpu_model *models, int32_t nchains, const sgpu_data
             return cleanup(rc);
         }
     }
-    int total_lt = 0, total_qb = 0;
-    for (int ci = 0; ci < nchains; ci++) {
-        total_lt += (int)cdiv(models[ci].L, kZLoci);
-        total_qb += (int)cdiv(models[ci].N, kNT);
-    }
+    const BatchTiles bt = batch_tiles(models, data, data_of_chain, nchains, z_target,
+                                      metro_target);
     c->chains.resize(nchains);
     std::vector<double> lam0(nchains);
     for (int ci = 0; ci < nchains; ci++) {
@@ -819,8 +870,7 @@ int sgpu_create_batch(const sgpu_model *models, int32_t nchains, const sgpu_data
         c->any_alpha |= h.m.inferalpha != 0;
         c->any_lambda |= h.m.inferlambda != 0;
         c->any_freqscorr |= h.m.freqscorr != 0;
-        if ((rc = setup_chain(c, h, c->data[h.dataset], total_lt, total_qb, z_target,
-                              metro_target))) {
+        if ((rc = setup_chain(c, h, c->data[h.dataset], bt))) {
             return cleanup(rc);
         }
     }

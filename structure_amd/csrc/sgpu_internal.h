@@ -45,6 +45,7 @@ struct ChainDesc {
     int fmt;                       // 1 = 2-bit SNP words, 2 = byte per copy
     int Lpad, G;                   // padded loci per word row, word rows
     int n_lt, n_it, rows_per_tile; // UpdateZ tiles: loci tiles, individual tiles, rows per tile
+    int zt_loci, z_chunk;          // loci per UpdateZ block; byte path: loci staged in LDS at once
     int q_blocks, l_blocks;        // cdiv(N, 256), cdiv(L, 256)
     int n_ranges, loci_per_range;  // Metropolis locus ranges
     int nhist, nloci_nonzero;      // allele-count histogram size, loci with alleles
