@@ -305,6 +305,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-sweeps", type=int, default=10)
+    ap.add_argument("--tune", action="store_true",
+                    help="search the tile shapes (structure_amd.tune) before timing")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5", "c1batch"],
                     help="c2 (default, the headline line), c3, c5 or c1batch")
     args, _ = ap.parse_known_args()
@@ -331,7 +333,12 @@ def main():
           flush=True)
     model = sa.make_model(N, L, K, 2, freqscorr=1, inferalpha=1, inferlambda=1, metrofreq=10,
                           burnin=args.warmup, numreps=args.steps, seed=args.seed, chain=0)
-    ch = sa.Chain(model, sa.GENO_SNP2, d["words"], d["nall"])
+    tuning = None
+    if args.tune:
+        from structure_amd import tune
+        tuning, _ = tune.autotune([model], [(sa.GENO_SNP2, d["words"], d["nall"])], cache=None,
+                                  log=lambda s: print("bench: " + s, file=sys.stderr, flush=True))
+    ch = sa.Batch([model], [(sa.GENO_SNP2, d["words"], d["nall"])], tuning=tuning)
     ch.init()
     ch.run(0, args.warmup)
     ch.synchronize()
@@ -343,7 +350,7 @@ def main():
     ch.run(args.warmup, args.steps)
     ch.synchronize()
     dt = time.perf_counter() - t0
-    like = ch.get(sa.LIKE)
+    like = ch.get_chain(0, sa.LIKE)
     # per-phase kernel durations: the same sweeps launched directly, with HIP
     # events around every phase on the chain's stream
     nt = min(args.steps, 100)
@@ -392,6 +399,8 @@ def main():
         "sweep_hbm_frac": round(sweep_bytes(N, L, K, 2) * sweeps_per_s / 1e9 / PEAK_HBM_GBS, 5),
         "phase_ms_per_sweep": {p: round(v[0] / nt, 4) for p, v in timing.items()},
         "phase_timing_sweeps": nt,
+        "tuning": ({n: getattr(tuning, n) for n, _ in sa.Tuning._fields_} if tuning else
+                   "library defaults"),
         "lnP_last": like[0],
     }
     if not args.no_batch:

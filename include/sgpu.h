@@ -111,14 +111,23 @@ int sgpu_pack_byte(const int8_t *geno, int32_t N, int32_t L, uint32_t *out, size
  * allocates device state and uploads the packed genotypes. */
 int sgpu_create(const sgpu_model *m, const sgpu_data *d, sgpu_ctx **out);
 
-/* Tile targets of the batched launches (0 = library default, about 2048
- * blocks, 8 per CU): UpdateZ individual tiles and Metropolis locus ranges are
- * chosen so that one launch over the whole batch has about this many blocks.
- * The counts do not depend on them; lnP(D) and the Metropolis log-ratios are
- * FP64 sums whose grouping does (differences in the last bits). */
+/* Tile shapes of the batched launches; 0 selects the library default.
+ *   z_blocks      blocks per 2-bit UpdateZ launch over the whole batch (sets the
+ *                 individuals per block; default 4096)
+ *   metro_blocks  blocks per Metropolis-ratio launch (sets the loci per block;
+ *                 default 1024)
+ *   byte_loci     loci per byte-format UpdateZ block (default 64 below 16384 loci,
+ *                 else 256; at most 256)
+ *   byte_chunk    loci whose K x A frequencies and counts are staged in LDS at
+ *                 once by the byte-format UpdateZ (default: as many as fit 60 KiB)
+ * The draws and counts do not depend on them; lnP(D) and the Metropolis
+ * log-ratios are FP64 sums whose grouping does (differences in the last bits).
+ * structure_amd/tune.py searches them per shape. */
 typedef struct {
     int32_t z_blocks;
     int32_t metro_blocks;
+    int32_t byte_loci;
+    int32_t byte_chunk;
 } sgpu_tuning;
 
 /* A batch of chains in one context (batched form of InitCLDict; the reference

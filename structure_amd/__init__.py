@@ -52,7 +52,8 @@ class Tuning(C.Structure):
     """Mirror of ``sgpu_tuning``: block targets of the batched UpdateZ and
     Metropolis launches (0 = library default)."""
 
-    _fields_ = [("z_blocks", C.c_int32), ("metro_blocks", C.c_int32)]
+    _fields_ = [("z_blocks", C.c_int32), ("metro_blocks", C.c_int32),
+                ("byte_loci", C.c_int32), ("byte_chunk", C.c_int32)]
 
 
 # Defaults of the shipped extraparams / PresetValues (extraparams:1-120, params.c:571-670)

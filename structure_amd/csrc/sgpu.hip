@@ -56,11 +56,15 @@ int fail(int code, const char *fmt, ...)
 
 size_t cdiv(size_t a, size_t b) { return (a + b - 1) / b; }
 
-// Default block targets of the tiled phases (about 8 blocks per CU on 256 CUs).
-constexpr int kZBlocksDefault = 2048;
-constexpr int kMetroBlocksDefault = 2048;
+// Default block targets of the tiled phases (16 and 4 blocks per CU on 256 CUs;
+// the 2-bit UpdateZ is measured fastest near 4096 at config 2, where 2048
+// leaves the second round of blocks on the CUs a third full).
+constexpr int kZBlocksDefault = 4096;
+constexpr int kMetroBlocksDefault = 1024;
 // LDS per UpdateZ block for the per-individual counts (keeps 3 blocks per CU).
 constexpr size_t kZLdsMax = 48 * 1024;
+// Most loci one Metropolis-ratio block walks.
+constexpr long long kMetroMaxLoci = 2048;
 // LDS per byte-format UpdateZ block for the staged frequencies and counts.
 constexpr size_t kByteLdsMax = 60 * 1024;
 
@@ -563,6 +567,8 @@ int upload_dataset(sgpu_ctx *c, Dataset &ds, const sgpu_data *d)
 struct BatchTiles {
     int snp_rows;   // 16-individual word rows per 2-bit UpdateZ block (before the LDS cap)
     int metro_lpr;  // loci per Metropolis block (multiple of 4)
+    int byte_loci;  // loci per byte-format UpdateZ block (0 = by data-set size)
+    int byte_chunk; // loci staged in LDS at once by the byte path (0 = as many as fit)
 };
 
 BatchTiles batch_tiles(const sgpu_model *models, const sgpu_data *data,
@@ -577,11 +583,13 @@ BatchTiles batch_tiles(const sgpu_model *models, const sgpu_data *data,
         }
         metro_work += (long long)cdiv(m.N, kNT) * m.L;
     }
-    BatchTiles t;
+    BatchTiles t{};
     t.snp_rows = (int)std::max<long long>(1, snp_work / std::max(z_target, 1));
-    const long long lpr = std::max<long long>(4, (metro_work + metro_target - 1) /
-                                                     std::max(metro_target, 1));
-    t.metro_lpr = (int)std::min<long long>((lpr + 3) / 4 * 4, 1 << 30);
+    // at most kMetroMaxLoci loci per block, so a large batch keeps many blocks
+    const long long lpr = std::min<long long>(
+        kMetroMaxLoci,
+        std::max<long long>(4, (metro_work + metro_target - 1) / std::max(metro_target, 1)));
+    t.metro_lpr = (int)((lpr + 3) / 4 * 4);
     return t;
 }
 
@@ -616,13 +624,17 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         // one individual pair per lane, 256 pairs per block; short data sets take
         // 64-locus tiles so that small chains still spread over many blocks; a
         // chunk of loci's K x A frequencies (FP64) and counts (u32) sit in LDS
-        d.zt_loci = L < 16384 ? 64 : kZLoci;
+        d.zt_loci = bt.byte_loci > 0 ? std::min(bt.byte_loci, kZLoci) : (L < 16384 ? 64 : kZLoci);
         d.n_lt = (int)cdiv(L, d.zt_loci);
         d.rows_per_tile = kByteRows;
         d.n_it = (int)cdiv(d.G, kByteRows);
         const size_t per_locus = (size_t)K * A * (sizeof(double) + sizeof(uint32_t));
-        d.z_chunk = (int)std::max<size_t>(1, std::min<size_t>(kByteLdsMax / per_locus,
-                                                              (size_t)d.zt_loci));
+        size_t chunk = std::max<size_t>(1, std::min<size_t>(kByteLdsMax / per_locus,
+                                                            (size_t)d.zt_loci));
+        if (bt.byte_chunk > 0) {
+            chunk = std::min<size_t>(chunk, (size_t)bt.byte_chunk);
+        }
+        d.z_chunk = (int)chunk;
         h.z_lds = (size_t)d.z_chunk * per_locus;
     }
     d.q_blocks = (int)cdiv(N, kNT);
@@ -815,6 +827,11 @@ int sgpu_create_batch(const sgpu_model *models, int32_t nchains, const sgpu_data
             }
         }
     }
+    if (tuning && (tuning->byte_loci < 0 || tuning->byte_loci > kZLoci || tuning->byte_chunk < 0 ||
+                   tuning->z_blocks < 0 || tuning->metro_blocks < 0)) {
+        return fail(SGPU_ERR_ARG, "tuning: byte_loci must lie in [0, %d], the rest be >= 0",
+                    kZLoci);
+    }
     const int z_target = (tuning && tuning->z_blocks > 0) ? tuning->z_blocks : kZBlocksDefault;
     const int metro_target =
         (tuning && tuning->metro_blocks > 0) ? tuning->metro_blocks : kMetroBlocksDefault;
@@ -858,8 +875,9 @@ int sgpu_create_batch(const sgpu_model *models, int32_t nchains, const sgpu_data
             return cleanup(rc);
         }
     }
-    const BatchTiles bt = batch_tiles(models, data, data_of_chain, nchains, z_target,
-                                      metro_target);
+    BatchTiles bt = batch_tiles(models, data, data_of_chain, nchains, z_target, metro_target);
+    bt.byte_loci = tuning ? tuning->byte_loci : 0;
+    bt.byte_chunk = tuning ? tuning->byte_chunk : 0;
     c->chains.resize(nchains);
     std::vector<double> lam0(nchains);
     for (int ci = 0; ci < nchains; ci++) {

@@ -80,15 +80,19 @@ def test_batch_tuning_changes_tiles_not_results():
     models = _models(dsets, common)
     a = sa.Batch(models, [d[:3] for d in dsets], [c[0] for c in CHAINS])
     b = sa.Batch(models, [d[:3] for d in dsets], [c[0] for c in CHAINS],
-                 tuning=sa.Tuning(z_blocks=1, metro_blocks=1))
-    for x in (a, b):
+                 tuning=sa.Tuning(z_blocks=1, metro_blocks=1, byte_loci=32, byte_chunk=3))
+    c = sa.Batch(models, [d[:3] for d in dsets], [c[0] for c in CHAINS],
+                 tuning=sa.Tuning(z_blocks=100000, metro_blocks=100000, byte_loci=256,
+                                  byte_chunk=1))
+    for x in (a, b, c):
         x.init()
         x.run(0, 8)
     for ci in range(len(CHAINS)):
-        for sel in EXACT:
-            assert np.array_equal(a.get_chain(ci, sel), b.get_chain(ci, sel)), (ci, sel)
-        np.testing.assert_allclose(a.get_chain(ci, sa.LIKE), b.get_chain(ci, sa.LIKE),
-                                   rtol=1e-12)
+        for other in (b, c):
+            for sel in EXACT:
+                assert np.array_equal(a.get_chain(ci, sel), other.get_chain(ci, sel)), (ci, sel)
+            np.testing.assert_allclose(a.get_chain(ci, sa.LIKE), other.get_chain(ci, sa.LIKE),
+                                       rtol=1e-12)
 
 
 def test_batch_state_is_per_chain():

@@ -176,3 +176,12 @@ def test_host_struct_layouts_match_the_headers(tmp_path):
     want = [C.sizeof(host.Params), host.Params.seed.offset, host.Params.emerror.offset,
             C.sizeof(host.Input), C.sizeof(host.Results)]
     assert [int(x) for x in got] == want
+
+
+def test_tuning_is_validated_before_touching_the_device():
+    d = sa.synth_snp(20, 30, Ktrue=2, seed=1)
+    m = sa.make_model(20, 30, 2, 2)
+    for bad in (sa.Tuning(byte_loci=-1), sa.Tuning(byte_loci=512), sa.Tuning(z_blocks=-5)):
+        with pytest.raises(sa.SgpuError) as e:
+            sa.Batch([m], [(sa.GENO_SNP2, d["words"], d["nall"])], tuning=bad)
+        assert e.value.code == sa.ERR_ARG
