@@ -23,9 +23,8 @@
 
 namespace sgpu {
 
-#ifndef SGPU_ZVAR
-#define SGPU_ZVAR 1
-#endif
+// Lane mask of a predicate (the condition's own mask, no VGPR round trip).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // v + (bit of this lane in mask): one carry-in add, the mask staying in SGPRs.
 __device__ __forceinline__ uint32_t add_lane_bit(uint32_t v, uint64_t mask)
@@ -45,10 +44,27 @@ __device__ __forceinline__ void collect_add(uint32_t &v, uint32_t value)
 
 // Running product of clamped terms with exponent bookkeeping (CalcLikeInd's
 // running product, CalcLike.c:113-126, without the periodic logs).
+// max(t, c) for the finite non-negative terms here, without the NaN
+// canonicalisation the compiler adds to fmax.
+__device__ __forceinline__ double max_raw(double t, double c)
+{
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(t), "v"(c));
+    return r;
+}
+
 struct LikeAcc {
     double prod = 1.0;
     int expo = 0;
     __device__ __forceinline__ void mul(double t) { prod *= (t > kSqrtUnder) ? t : kSqrtUnder; }
+    // both copies of one individual, branch-free: terms clamped at sqrt(UNDERFLO),
+    // a missing individual contributes 1
+    __device__ __forceinline__ void mul_pair(double t0, double t1, bool valid)
+    {
+        const double c = kSqrtUnder;
+        const double tt = max_raw(t0, c) * max_raw(t1, c);
+        prod *= valid ? tt : 1.0;
+    }
     __device__ __forceinline__ void renorm()
     {
         int e;
@@ -131,6 +147,17 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     uint32_t word = geno[(size_t)row0 * Lpad + l];
     for (int g = row0; g < row1; g++) {
         const uint32_t next = (g + 1 < row1) ? geno[(size_t)(g + 1) * Lpad + l] : 0u;
+        {
+            // this row's valid copies and allele-1 copies at the lane's locus, from the
+            // word's bit pattern (code 3 = missing; only individuals < N count)
+            const int nreal = min(16, N - g * 16);
+            const uint32_t live =
+                nreal >= 16 ? 0x55555555u : (((1u << (2 * nreal)) - 1u) & 0x55555555u);
+            const uint32_t lo = word & 0x55555555u, hi = (word >> 1) & 0x55555555u;
+            const uint32_t vmask = ~(lo & hi) & live;
+            Ttot += 2u * (uint32_t)__popc(vmask);
+            A1tot += (uint32_t)__popc(lo & ~hi & vmask) + 2u * (uint32_t)__popc(hi & ~lo & vmask);
+        }
         uint32_t col[K];   // collector: lane t holds the counts of individual 16g+t
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -156,12 +183,9 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                 const bool x1 = code != 0u;
                 const uint32_t wa = h ? w.z : w.x;
                 const uint32_t wb = h ? w.w : w.y;
-                Ttot += valid;
-                Ttot += valid;
-                A1tot += valid && x0;
-                A1tot += valid && x1;
+                const uint64_t vm = ballot(valid);
                 uint32_t nk[K];   // this individual's counts (wave-uniform)
-                nk[0] = 2u * (uint32_t)__popcll(__ballot(valid));
+                nk[0] = 2u * (uint32_t)__popcll(vm);
                 int z0 = 0, z1 = 0;
                 if (INIT) {
                     const uint32_t u0 = (uint32_t)(((uint64_t)wa * (uint32_t)K) >> 32);
@@ -174,7 +198,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                         T[k] += m1;
                         A1[k] += m0 && x0;
                         A1[k] += m1 && x1;
-                        nk[k + 1] = (uint32_t)(__popcll(__ballot(m0)) + __popcll(__ballot(m1)));
+                        nk[k + 1] = (uint32_t)(__popcll(ballot(m0)) + __popcll(ballot(m1)));
                     }
                 } else {
                     double pre0[K], pre1[K];
@@ -184,14 +208,15 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                         const double q = Q[(size_t)i * K + k];
                         const double c0 = q * P0[k];
                         const double c1 = q * P1[k];
-                        s0 = s0 + c0;
-                        s1 = s1 + c1;
+                        // the reference's running sum starts at 0.0; 0.0 + c == c for the
+                        // non-negative products here, so the first term starts it
+                        s0 = k ? s0 + c0 : c0;
+                        s1 = k ? s1 + c1 : c1;
                         pre0[k] = s0;
                         pre1[k] = s1;
                     }
-                    if (LIKE && valid) {
-                        acc.mul(x0 ? s1 : s0);
-                        acc.mul(x1 ? s1 : s0);
+                    if (LIKE) {
+                        acc.mul_pair(x0 ? s1 : s0, x1 ? s1 : s0, valid);
                     }
                     // r = (w * 2^-32) * S, formed as w * (S * 2^-32): the power-of-two
                     // scaling is exact, so both orders round the same product once.
@@ -199,14 +224,13 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                     const double h0 = s0 * 0x1p-32, h1 = s1 * 0x1p-32;
                     const double r0 = valid ? (double)wa * (x0 ? h1 : h0) : -1.0;
                     const double r1 = valid ? (double)wb * (x1 ? h1 : h0) : -1.0;
-#if SGPU_ZVAR
                     // both allele sequences are compared and the copy's allele picks
                     // the lane mask in scalar registers
-                    const uint64_t xm0 = __ballot(x0), xm1 = __ballot(x1);
+                    const uint64_t xm0 = ballot(x0), xm1 = ballot(x1);
 #pragma unroll
                     for (int k = 0; k < K - 1; k++) {
-                        const uint64_t a0 = __ballot(pre0[k] < r0), b0 = __ballot(pre1[k] < r0);
-                        const uint64_t a1 = __ballot(pre0[k] < r1), b1 = __ballot(pre1[k] < r1);
+                        const uint64_t a0 = ballot(pre0[k] < r0), b0 = ballot(pre1[k] < r0);
+                        const uint64_t a1 = ballot(pre0[k] < r1), b1 = ballot(pre1[k] < r1);
                         const uint64_t m0 = (xm0 & b0) | (~xm0 & a0);   // z0 > k
                         const uint64_t m1 = (xm1 & b1) | (~xm1 & a1);   // z1 > k
                         T[k] = add_lane_bit(add_lane_bit(T[k], m0), m1);
@@ -217,22 +241,6 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                             z1 += (int)((m1 >> lane) & 1u);
                         }
                     }
-#else
-#pragma unroll
-                    for (int k = 0; k < K - 1; k++) {
-                        const bool m0 = x0 ? (pre1[k] < r0) : (pre0[k] < r0);   // z0 > k
-                        const bool m1 = x1 ? (pre1[k] < r1) : (pre0[k] < r1);   // z1 > k
-                        T[k] += m0;
-                        T[k] += m1;
-                        A1[k] += m0 && x0;
-                        A1[k] += m1 && x1;
-                        nk[k + 1] = (uint32_t)(__popcll(__ballot(m0)) + __popcll(__ballot(m1)));
-                        if (ZOUT) {
-                            z0 += m0;
-                            z1 += m1;
-                        }
-                    }
-#endif
                 }
                 // park the counts in lane t of the collector: one add per value,
                 // executed by that lane only
