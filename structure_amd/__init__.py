@@ -105,6 +105,7 @@ def lib() -> C.CDLL:
             "sgpu_synchronize": (C.c_int, [vp]),
             "sgpu_set_timing": (C.c_int, [vp, i32]),
             "sgpu_get_timing": (C.c_int, [vp, vp, vp]),
+            "sgpu_eval_math": (C.c_int, [i32, vp, vp, C.c_int64]),
             "sgpu_last_error": (C.c_char_p, []),
             "sgpu_version": (C.c_char_p, []),
             "sgpu_synth_snp": (C.c_int, [i32, i32, i32, C.c_double, C.c_double, C.c_double,
@@ -129,6 +130,17 @@ def _check(rc):
 def _vp(a: np.ndarray):
     assert a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(C.c_void_p)
+
+
+FN_LOG, FN_COS, FN_LGAMMA = 0, 1, 2
+
+
+def eval_math(fn: int, x) -> np.ndarray:
+    """The device's own log / cos / lgamma over an array (diagnostics)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    _check(lib().sgpu_eval_math(fn, _vp(x), _vp(y), x.size))
+    return y
 
 
 # ---- genotype packing and synthetic data -------------------------------------

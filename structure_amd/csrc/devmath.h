@@ -42,13 +42,14 @@ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
         k1 += 0xBB67AE85u;
     }
 #else
-    // 32x32->64 products written as one 64-bit multiply each (v_mad_u64_u32)
+    // 32x32->64 products written as one 64-bit multiply each (v_mad_u64_u32);
+    // the three-input XORs as one v_bitop3_b32 each (truth table 0x96)
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -70,10 +71,90 @@ __device__ __forceinline__ double uz(uint32_t w) { return (double)w * 0x1p-32; }
 // uniform in (0,1), exact: rnd() semantics (ran.c:152-161)
 __device__ __forceinline__ double uo(uint32_t w) { return ((double)w + 0.5) * 0x1p-32; }
 
+// ln x for the sampler and the Metropolis log-ratios.  x = m 2^e with m in
+// [sqrt(1/2), sqrt(2)), f = m - 1 (exact) and s = f / (2 + f), so that
+// ln(1 + f) = 2 atanh(s) = f - s (f - R(s^2)), R(z) = sum_{j>=1} 2 z^j / (2j + 1)
+// (the Taylor series of 2 atanh(s) / s - 2).  |s| <= 0.1716, so ten terms
+// leave a truncation error below 2^-58 of the result.  ln 2 is split into a
+// 32-bit head (e * head exact for every exponent) and a tail.  The reciprocal
+// of 2 + f in [1.29, 2.42] is the hardware estimate refined by two Newton
+// steps.  Error about 1 ulp; 0 -> -inf, +inf -> +inf, negative or NaN -> NaN.
+// About 35 instructions instead of the library log's double-double
+// evaluation (over 100).
+__device__ __forceinline__ double log_d(double x)
+{
+    double m = __builtin_amdgcn_frexp_mant(x);
+    int e = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0.7071067811865476) {
+        m = m + m;
+        e -= 1;
+    }
+    const double f = m - 1.0;
+    const double den = 2.0 + f;
+    double y = __builtin_amdgcn_rcp(den);
+    y = fma(y, fma(-den, y, 1.0), y);
+    y = fma(y, fma(-den, y, 1.0), y);
+    const double s = f * y;
+    const double z = s * s;
+    double R = 0.09523809523809523;   // 2/21
+    R = fma(R, z, 0.10526315789473684);
+    R = fma(R, z, 0.11764705882352941);
+    R = fma(R, z, 0.13333333333333333);
+    R = fma(R, z, 0.15384615384615385);
+    R = fma(R, z, 0.18181818181818182);
+    R = fma(R, z, 0.2222222222222222);
+    R = fma(R, z, 0.2857142857142857);
+    R = fma(R, z, 0.4);
+    R = fma(R, z, 0.6666666666666666);   // 2/3
+    R = R * z;
+    const double lm = fma(-s, f - R, f);
+    const double de = (double)e;
+    const double r = fma(de, 0.6931471803691238, fma(de, 1.9082149292705877e-10, lm));
+    if (x > 0.0 && x < __builtin_inf()) {
+        return r;
+    }
+    return x == 0.0 ? -__builtin_inf() : (x == __builtin_inf() ? x : __builtin_nan(""));
+}
+
+// cos a for 0 <= a <= 8 (the Box-Muller angle).  a = q pi/2 + r with
+// q = rint(2a/pi) and r from a two-part pi/2 (Cody-Waite; the head product is
+// exact under the fused multiply-add, so r keeps its relative accuracy next
+// to the zeros of cos); cos r and sin r by their Taylor series through r^16
+// and r^17 (|r| <= pi/4: truncation below 2^-60), quadrant by q mod 4.
+// Error about 1 ulp.
+__device__ __forceinline__ double cos_d(double a)
+{
+    const double q = rint(a * 0.6366197723675814);
+    double r = fma(-q, 1.5707963267948966, a);
+    r = fma(-q, 6.123233995736766e-17, r);
+    const int n = (int)q & 3;
+    const double z = r * r;
+    double c = 4.779477332387385e-14;   // 1/16!
+    c = fma(c, z, -1.1470745597729725e-11);
+    c = fma(c, z, 2.08767569878681e-09);
+    c = fma(c, z, -2.755731922398589e-07);
+    c = fma(c, z, 2.48015873015873e-05);
+    c = fma(c, z, -0.001388888888888889);
+    c = fma(c, z, 0.041666666666666664);
+    c = fma(c, z, -0.5);
+    c = fma(c, z, 1.0);
+    double t = 2.8114572543455206e-15;   // 1/17!
+    t = fma(t, z, -7.647163731819816e-13);
+    t = fma(t, z, 1.6059043836821613e-10);
+    t = fma(t, z, -2.505210838544172e-08);
+    t = fma(t, z, 2.7557319223985893e-06);
+    t = fma(t, z, -0.0001984126984126984);
+    t = fma(t, z, 0.008333333333333333);
+    t = fma(t, z, -0.16666666666666666);
+    t = fma(t * z, r, r);
+    const double v = (n & 1) ? t : c;
+    return (n == 1 || n == 2) ? -v : v;
+}
+
 __device__ __forceinline__ double box_muller(uint32_t w0, uint32_t w1)
 {
-    double r = sqrt(-2.0 * log(uo(w0)));
-    return r * cos(6.283185307179586 * uo(w1));
+    double r = sqrt(-2.0 * log_d(uo(w0)));
+    return r * cos_d(6.283185307179586 * uo(w1));
 }
 
 __device__ __forceinline__ double normal_draw(uint32_t seed, uint32_t chain, uint32_t c0,
@@ -110,10 +191,10 @@ __device__ __forceinline__ double gamma_draw(double a, uint32_t seed, uint32_t c
         v = v * v * v;
         const double u = uo(w.z);
         const double x2 = x * x;
-        if (u < 1.0 - 0.0331 * (x2 * x2) || log(u) < 0.5 * x2 + d * (1.0 - v + log(v))) {
+        if (u < 1.0 - 0.0331 * (x2 * x2) || log_d(u) < 0.5 * x2 + d * (1.0 - v + log_d(v))) {
             double g = d * v;
             if (a < 1.0) {
-                g = g * exp(log(uo(w.w)) / a);
+                g = g * exp(log_d(uo(w.w)) / a);
             }
             return g;
         }
@@ -138,7 +219,7 @@ __device__ __forceinline__ double lgamma_lanczos_terms(double z)
     s += c2 / den; den -= 1.0;
     s += c1 / den;
     s += 0.9999999999995183;
-    return log(s) + 0.9189385332046727 - (z + 6.5) + (z - 0.5) * log(z + 6.5);
+    return log_d(s) + 0.9189385332046727 - (z + 6.5) + (z - 0.5) * log_d(z + 6.5);
 }
 
 // ln Gamma(z), z > 0: Lanczos g = 7, n = 9 with the published coefficients,
@@ -163,7 +244,7 @@ __device__ __forceinline__ double lgamma_lanczos(double z)
         e -= 1.0;
     }
     const double s = n / d + 0.9999999999995183;
-    return log(s) + 0.9189385332046727 - (z + 6.5) + (z - 0.5) * log(z + 6.5);
+    return log_d(s) + 0.9189385332046727 - (z + 6.5) + (z - 0.5) * log_d(z + 6.5);
 }
 
 // Wave-level (64 lanes) and block-level FP64 sums with a fixed reduction tree.

@@ -26,13 +26,14 @@ constexpr int kNT = 256;   // threads per block for the update kernels
     (void)bx
 
 // ---------------------------------------------------------------------------
-// UpdateP (UpdateP.c:106-199): one thread per (locus, pop).  Dirichlet
+// UpdateP (UpdateP.c:106-199): one thread per (locus, pop, allele), a block
+// holding whole (locus, pop) groups (A * (256 / A) threads).  Dirichlet
 // parameters from the allele counts of the last UpdateZ (summed over the
-// individual-tile partials), draw via LogRDirichletDisc's rule
+// individual-tile partials, coalesced), one gamma draw per thread, then the
+// group's sum in allele order from LDS and LogRDirichletDisc's rule
 // (ran.c:1247-1272) including the sum < UNDERFLO uniform fallback.
 // Fused: PSum += P (output.c:673-677) when collecting.
 // ---------------------------------------------------------------------------
-template <int ASTORE>
 __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ cds,
                                                    const BlockRef *__restrict__ map,
                                                    const uint32_t *__restrict__ repp, int collect)
@@ -46,57 +47,43 @@ __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ 
     const SGPU_G double *lambda = g(C.lambda);
     SGPU_G double *P = g(C.P);
     SGPU_G double *PSum = g(C.PSum);
-    const int n_it = C.n_it;
-    const int idx = bx * kNT + threadIdx.x;
-    if (idx >= m.L * m.K) {
+    __shared__ double s_x[kNT];
+    const int A = m.A;
+    const int gpb = kNT / A;   // (locus, pop) groups per block
+    const int grp = (int)threadIdx.x / A, aa = (int)threadIdx.x - grp * A;
+    const int idx = bx * gpb + grp;   // flat (locus, pop)
+    const bool live = grp < gpb && idx < m.L * m.K;
+    const int l = live ? idx / m.K : 0, k = live ? idx - l * m.K : 0;
+    const int na = live ? nall[l] : 0;
+    const bool act = aa < na;
+    const size_t pos = (size_t)idx * A + aa;
+    double x = 0.0;
+    if (act) {
+        const size_t stride = (size_t)m.L * m.K * A;
+        int n = 0;
+        for (int it = 0; it < C.n_it; it++) {
+            n += nApart[it * stride + pos];
+        }
+        const double Fk = F[k];
+        const double par = m.freqscorr ? eps[(size_t)l * A + aa] * (1.0 - Fk) / Fk + (double)n
+                                       : lambda[k] + (double)n;
+        x = gamma_draw(par, m.seed, m.chain, rep, (uint32_t)l, ((uint32_t)k << 16) | (uint32_t)aa,
+                       PH_P);
+    }
+    s_x[threadIdx.x] = x;
+    __syncthreads();
+    if (!act) {
         return;
     }
-    const int l = idx / m.K, k = idx % m.K;
-    const int na = nall[l];
-    const size_t base = ((size_t)l * m.K + k) * m.A;
-    const size_t stride = (size_t)m.L * m.K * m.A;
-    double gd[ASTORE > 0 ? ASTORE : 1];
     double sum = 0.0;
-    const double Fk = F[k];
-    for (int aa = 0; aa < na; aa++) {
-        int n = 0;
-        for (int it = 0; it < n_it; it++) {
-            n += nApart[it * stride + base + aa];
-        }
-        double par;
-        if (m.freqscorr) {
-            par = eps[(size_t)l * m.A + aa] * (1.0 - Fk) / Fk + (double)n;
-        } else {
-            par = lambda[k] + (double)n;
-        }
-        const double x = gamma_draw(par, m.seed, m.chain, rep, (uint32_t)l,
-                                    ((uint32_t)k << 16) | (uint32_t)aa, PH_P);
-        if (ASTORE > 0) {
-            gd[aa] = x;
-        }
-        sum += x;
+    const int g0 = (int)threadIdx.x - aa;
+    for (int j = 0; j < na; j++) {   // allele order, as the host loop adds them
+        sum += s_x[g0 + j];
     }
-    const bool uniform = sum < kUnderflo;
-    for (int aa = 0; aa < na; aa++) {
-        double x;
-        if (ASTORE > 0) {
-            x = gd[aa];
-        } else {
-            // recompute the identical draw (same counter) instead of storing it
-            int n = 0;
-            for (int it = 0; it < n_it; it++) {
-                n += nApart[it * stride + base + aa];
-            }
-            const double par = m.freqscorr ? eps[(size_t)l * m.A + aa] * (1.0 - Fk) / Fk + (double)n
-                                           : lambda[k] + (double)n;
-            x = gamma_draw(par, m.seed, m.chain, rep, (uint32_t)l,
-                           ((uint32_t)k << 16) | (uint32_t)aa, PH_P);
-        }
-        const double p = uniform ? 1.0 / (double)na : x / sum;
-        P[base + aa] = p;
-        if (collect) {
-            PSum[base + aa] += p;
-        }
+    const double p = sum < kUnderflo ? 1.0 / (double)na : x / sum;
+    P[pos] = p;
+    if (collect) {
+        PSum[pos] += p;
     }
 }
 
@@ -137,13 +124,13 @@ __device__ __forceinline__ void finish_q(const ModelArgs &m, int bx, int nrows, 
             if (collect) {
                 QSum[(size_t)i * m.K + k] += q[k];
             }
-            tot += log(q[k] > kSqrtUnder ? q[k] : kSqrtUnder);
+            tot += log_d(q[k] > kSqrtUnder ? q[k] : kSqrtUnder);
         }
     }
     for (int k = 0; k <= m.K; k++) {
         double v = 0.0;
         if (active) {
-            v = (k < m.K) ? log(q[k]) : tot;
+            v = (k < m.K) ? log_d(q[k]) : tot;
         }
         const double s = block_sum<NT>(v, s_red);
         if (threadIdx.x == 0) {
@@ -236,10 +223,10 @@ __global__ __launch_bounds__(kNT) void k_logq_stats(const ChainDesc *__restrict_
     }
     double tot = 0.0;
     for (int k = 0; k < m.K; k++) {
-        tot += active ? log(q[k] > kSqrtUnder ? q[k] : kSqrtUnder) : 0.0;
+        tot += active ? log_d(q[k] > kSqrtUnder ? q[k] : kSqrtUnder) : 0.0;
     }
     for (int k = 0; k <= m.K; k++) {
-        const double v = active ? ((k < m.K) ? log(q[k]) : tot) : 0.0;
+        const double v = active ? ((k < m.K) ? log_d(q[k]) : tot) : 0.0;
         const double s = block_sum<kNT>(v, s_red);
         if (threadIdx.x == 0) {
             logq_part[(size_t)bx * (m.K + 1) + k] = s;
@@ -252,7 +239,7 @@ __global__ __launch_bounds__(kNT) void k_logq_stats(const ChainDesc *__restrict_
 
 // ---------------------------------------------------------------------------
 // UpdateQMetro (UpdateQ.c:199-307): proposal from the prior, accepted with
-// probability exp(sum over copies of log(TestQ.P) - log(Q.P))
+// probability exp(sum over copies of log_d(TestQ.P) - log_d(Q.P))
 // (CalcLikeIndDiff, CalcLike.c:281-345).  Pass 1: one thread per
 // individual and a locus range per grid row; each thread regenerates its
 // proposal from the stream and accumulates both products in FP64 with
@@ -341,7 +328,7 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__res
         }
     }
     if (active) {
-        delta_part[(size_t)by * m.N + i] = (log(pp) - log(pm)) + (double)(ep - em) * kLn2;
+        delta_part[(size_t)by * m.N + i] = (log_d(pp) - log_d(pm)) + (double)(ep - em) * kLn2;
     }
 }
 
@@ -409,7 +396,7 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__res
         pm = frexp(pm, &e);
         em += e;
     }
-    delta_part[(size_t)by * m.N + i] = (log(pp) - log(pm)) + (double)(ep - em) * kLn2;
+    delta_part[(size_t)by * m.N + i] = (log_d(pp) - log_d(pm)) + (double)(ep - em) * kLn2;
 }
 
 __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restrict__ cds,
@@ -450,7 +437,7 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restric
 // ---------------------------------------------------------------------------
 // UpdateAlpha (UpdateAlpha.c:189-253), one block per chain: sums the log-Q
 // statistics and runs the pop loop in thread 0.  LogProbQ(a) = (a-1) * sum
-// log(clamped Q) + (lgamma(K a) - K lgamma(a)) * N; LogProbQonepop as in the
+// log_d(clamped Q) + (lgamma(K a) - K lgamma(a)) * N; LogProbQonepop as in the
 // reference.  As there, logprobdiff is zeroed once and only reset by the
 // gamma prior.
 // ---------------------------------------------------------------------------
@@ -489,7 +476,7 @@ __global__ __launch_bounds__(kNT) void k_update_alpha(const ChainDesc *__restric
         const double newalpha = oldalpha + m.alphapropsd * z;
         if ((newalpha > 0) && ((newalpha < m.alphamax) || (!m.unifprioralpha))) {
             if (!m.unifprioralpha) {
-                logprobdiff = (m.alphapriora - 1) * log(newalpha / oldalpha) +
+                logprobdiff = (m.alphapriora - 1) * log_d(newalpha / oldalpha) +
                               (oldalpha - newalpha) / m.alphapriorb;
             }
             if (m.popalphas) {
@@ -545,16 +532,16 @@ __global__ __launch_bounds__(kNT) void k_lambda_partial(const ChainDesc *__restr
         if (na > 1) {
             if (m.popspecificlambda) {
                 for (int aa = 0; aa < na; aa++) {
-                    v += log(P[((size_t)l * m.K + s) * m.A + aa]);
+                    v += log_d(P[((size_t)l * m.K + s) * m.A + aa]);
                 }
             } else if (m.freqscorr) {
                 for (int aa = 0; aa < na; aa++) {
-                    v += log(eps[(size_t)l * m.A + aa]);
+                    v += log_d(eps[(size_t)l * m.A + aa]);
                 }
             } else {
                 for (int k = 0; k < m.K; k++) {
                     for (int aa = 0; aa < na; aa++) {
-                        v += log(P[((size_t)l * m.K + k) * m.A + aa]);
+                        v += log_d(P[((size_t)l * m.K + k) * m.A + aa]);
                     }
                 }
             }
@@ -661,8 +648,7 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
         if (a2 >= a1) {
             a2 += 1;
         }
-        const double invsqrtn = pow((double)m.N, -0.5);
-        d = 0.0 + uz(w.z) * (invsqrtn - 0.0);
+        d = 0.0 + uz(w.z) * (m.invsqrtn - 0.0);
         e1 = e[a1];
         e2 = e[a2];
         inrange = (e1 + d < 1.0) && (e2 - d > 0.0);
@@ -673,8 +659,8 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
         s_term[1][threadIdx.x] = lgamma_lanczos(frac * e2);
         s_term[2][threadIdx.x] = lgamma_lanczos(frac * (e1 + d));
         s_term[3][threadIdx.x] = lgamma_lanczos(frac * (e2 - d));
-        s_term[4][threadIdx.x] = frac * d * log(P[((size_t)l * K + k) * m.A + a1]);
-        s_term[5][threadIdx.x] = frac * d * log(P[((size_t)l * K + k) * m.A + a2]);
+        s_term[4][threadIdx.x] = frac * d * log_d(P[((size_t)l * K + k) * m.A + a1]);
+        s_term[5][threadIdx.x] = frac * d * log_d(P[((size_t)l * K + k) * m.A + a2]);
     }
     __syncthreads();
     if (live && k == 0) {
@@ -691,7 +677,7 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
             }
             const double lam = lambda[0];
             if (lam != 1.0) {
-                sum += log(pow((e1 + d) * (e2 - d) / (e1) / (e2), (double)lam - 1.0));
+                sum += log_d(pow((e1 + d) * (e2 - d) / (e1) / (e2), (double)lam - 1.0));
             }
             if (uo(w.w) < exp(sum)) {
                 e[a1] = e1 + d;
@@ -707,8 +693,8 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Fst (UpdateFst.c:142-204).  Pass 1 draws the proposals once per block into
-// LDS and sums, per locus and proposal group, the allele terms of
+// Fst (UpdateFst.c:142-204).  Pass 1 (grid: locus blocks x proposal groups)
+// draws its group's proposal once per block and sums, per locus, the allele terms of
 // FlikeFreqs(new) - FlikeFreqs(old) (UpdateFst.c:22-51); pass 2 (one block
 // per chain) adds the prior ratio (FPriorDiff, :10-18) and the per-locus
 // lgamma(frac) terms and decides each group in its own thread.  ONEFST: one
@@ -736,36 +722,35 @@ __global__ __launch_bounds__(kNT) void k_fst_partial(const ChainDesc *__restrict
     const SGPU_G double *F = g(C.F);
     SGPU_G double *part = g(C.fst_part);
     __shared__ double s_red[kNT / 64];
-    __shared__ double s_new[kMaxK], s_old[kMaxK];
+    __shared__ double s_f[2];
     const int numpops1 = m.onefst ? 1 : m.K;
-    if ((int)threadIdx.x < numpops1) {   // the block's proposals, drawn once
-        s_old[threadIdx.x] = F[threadIdx.x];
-        s_new[threadIdx.x] = fst_proposal(m, s_old[threadIdx.x], threadIdx.x, rep, nullptr);
+    const int p1 = (int)br_.by;   // the proposal group of this block
+    if (threadIdx.x == 0) {       // the group's proposal, drawn once per block
+        s_f[0] = F[p1];
+        s_f[1] = fst_proposal(m, s_f[0], p1, rep, nullptr);
     }
     __syncthreads();
     const int l = bx * kNT + threadIdx.x;
     const int na = (l < m.L) ? nall[l] : 0;
-    for (int p1 = 0; p1 < numpops1; p1++) {
-        const double oldf = s_old[p1];
-        const double newf = s_new[p1];
-        double v = 0.0;
-        if (newf > 0.0 && newf < 1.0 && na > 0) {
-            const double nfr = (1.0 - newf) / newf;
-            const double ofr = (1.0 - oldf) / oldf;
-            const int p2end = m.onefst ? m.K : p1 + 1;
-            for (int p2 = p1; p2 < p2end; p2++) {
-                for (int aa = 0; aa < na; aa++) {
-                    const double ep = eps[(size_t)l * m.A + aa];
-                    const double lp = log(P[((size_t)l * m.K + p2) * m.A + aa]);
-                    v += (nfr * ep * lp - lgamma_lanczos(nfr * ep)) -
-                         (ofr * ep * lp - lgamma_lanczos(ofr * ep));
-                }
+    const double oldf = s_f[0];
+    const double newf = s_f[1];
+    double v = 0.0;
+    if (newf > 0.0 && newf < 1.0 && na > 0) {
+        const double nfr = (1.0 - newf) / newf;
+        const double ofr = (1.0 - oldf) / oldf;
+        const int p2end = m.onefst ? m.K : p1 + 1;
+        for (int p2 = p1; p2 < p2end; p2++) {
+            for (int aa = 0; aa < na; aa++) {
+                const double ep = eps[(size_t)l * m.A + aa];
+                const double lp = log_d(P[((size_t)l * m.K + p2) * m.A + aa]);
+                v += (nfr * ep * lp - lgamma_lanczos(nfr * ep)) -
+                     (ofr * ep * lp - lgamma_lanczos(ofr * ep));
             }
         }
-        const double t = block_sum<kNT>(v, s_red);
-        if (threadIdx.x == 0) {
-            part[(size_t)bx * numpops1 + p1] = t;
-        }
+    }
+    const double t = block_sum<kNT>(v, s_red);
+    if (threadIdx.x == 0) {
+        part[(size_t)bx * numpops1 + p1] = t;
     }
 }
 
@@ -803,7 +788,7 @@ __global__ __launch_bounds__(kNT) void k_fst_finalize(const ChainDesc *__restric
     const double oldf = F[p1];
     const double newf = fst_proposal(m, oldf, p1, rep, &uw);
     if (newf > 0.0 && newf < 1.0) {
-        double lpd = ((mu * mu / (sd * sd) - 1) * log(newf / oldf) + (oldf - newf) * mu / (sd * sd));
+        double lpd = ((mu * mu / (sd * sd) - 1) * log_d(newf / oldf) + (oldf - newf) * mu / (sd * sd));
         const double nfr = (1.0 - newf) / newf;
         const double ofr = (1.0 - oldf) / oldf;
         const int ngroup = m.onefst ? m.K : 1;

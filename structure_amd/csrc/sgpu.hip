@@ -110,7 +110,7 @@ struct sgpu_ctx {
     bool any_alpha = false, any_lambda = false, any_freqscorr = false;
     // launch plans
     std::vector<KGroup> z_groups, metro_groups;
-    Plan p_update_p8, p_update_p0, p_fold, p_gibbs, p_logq, p_accept, p_alpha, p_lam_part,
+    Plan p_update_p, p_fold, p_gibbs, p_logq, p_accept, p_alpha, p_lam_part,
         p_lam_fin, p_eps, p_fst_part, p_fst_fin, p_collect, p_init_eps;
     std::vector<Plan> p_logq_chain;   // per chain, after a host write of Q
     // device sweep index read by every kernel
@@ -173,6 +173,7 @@ ModelArgs model_args(const sgpu_model &m)
     a.alphapriorb = m.alphapriorb;
     a.fpriormean = m.fpriormean;
     a.fpriorsd = m.fpriorsd;
+    a.invsqrtn = pow((double)m.N, -0.5);
     a.seed = m.seed;
     a.chain = m.chain;
     return a;
@@ -419,6 +420,27 @@ struct PhaseTimer {
 
 __global__ void k_set_rep(uint32_t *repd, uint32_t v) { repd[0] = v; }
 __global__ void k_advance_rep(uint32_t *repd) { repd[0] += 1u; }
+
+// The sampler's device special functions over an array (sgpu_eval_math).
+__global__ void k_eval_math(int fn, const double *__restrict__ x, double *__restrict__ y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const double v = x[i];
+    y[i] = fn == SGPU_FN_LOG ? log_d(v) : (fn == SGPU_FN_COS ? cos_d(v) : lgamma_lanczos(v));
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) {
+            (void)hipFree(p);
+        }
+    }
+};
 
 // Make the device sweep index equal rep (no-op while capturing a graph, whose
 // kernels read the index that the previous replay advanced).
@@ -684,12 +706,14 @@ int upload_descriptors(sgpu_ctx *c)
 int build_plans(sgpu_ctx *c)
 {
     int rc;
-    auto L_K = [](const HostChain &h) { return std::make_pair((int)cdiv((size_t)h.m.L * h.m.K, kNT), 1); };
     auto one = [](const HostChain &) { return std::make_pair(1, 1); };
     auto qb = [](const HostChain &h) { return std::make_pair(h.d.q_blocks, 1); };
     auto lb = [](const HostChain &h) { return std::make_pair(h.d.l_blocks, 1); };
-    if ((rc = build_plan(c, c->p_update_p8, [](const HostChain &h) { return h.m.A <= 8; }, L_K)) ||
-        (rc = build_plan(c, c->p_update_p0, [](const HostChain &h) { return h.m.A > 8; }, L_K)) ||
+    if ((rc = build_plan(c, c->p_update_p, all_chains,
+                         [](const HostChain &h) {
+                             return std::make_pair(
+                                 (int)cdiv((size_t)h.m.L * h.m.K, (size_t)(kNT / h.m.A)), 1);
+                         })) ||
         (rc = build_plan(c, c->p_fold, all_chains,
                          [](const HostChain &h) {
                              return std::make_pair((int)cdiv((size_t)h.m.N * h.m.K, kNT), kRowSplit);
@@ -703,7 +727,10 @@ int build_plans(sgpu_ctx *c)
         (rc = build_plan(c, c->p_lam_fin, [](const HostChain &h) { return h.m.inferlambda != 0; }, one)) ||
         (rc = build_plan(c, c->p_eps, [](const HostChain &h) { return h.m.freqscorr != 0; },
                          [](const HostChain &h) { return std::make_pair((int)cdiv(h.m.L, kNT / h.m.K), 1); })) ||
-        (rc = build_plan(c, c->p_fst_part, [](const HostChain &h) { return h.m.freqscorr != 0; }, lb)) ||
+        (rc = build_plan(c, c->p_fst_part, [](const HostChain &h) { return h.m.freqscorr != 0; },
+                         [](const HostChain &h) {
+                             return std::make_pair(h.d.l_blocks, h.m.onefst ? 1 : h.m.K);
+                         })) ||
         (rc = build_plan(c, c->p_fst_fin, [](const HostChain &h) { return h.m.freqscorr != 0; }, one)) ||
         (rc = build_plan(c, c->p_collect, all_chains, one)) ||
         (rc = build_plan(c, c->p_init_eps, all_chains, lb))) {
@@ -1082,8 +1109,7 @@ int sgpu_update_p(sgpu_ctx *c, uint32_t rep, int32_t collect)
     }
     PhaseTimer timer(c, SGPU_PH_P);
     const uint32_t *r = c->repd;
-    if ((rc = launch(c, k_update_p<8>, c->p_update_p8, kNT, r, (int)collect)) ||
-        (rc = launch(c, k_update_p<0>, c->p_update_p0, kNT, r, (int)collect))) {
+    if ((rc = launch(c, k_update_p, c->p_update_p, kNT, r, (int)collect))) {
         return rc;
     }
     return SGPU_OK;
@@ -1323,6 +1349,26 @@ int sgpu_get_timing(sgpu_ctx *c, double *ms, int64_t *launches)
         c->t_ms[p] = 0.0;
         c->t_n[p] = 0;
     }
+    return SGPU_OK;
+}
+
+int sgpu_eval_math(int32_t fn, const double *x, double *y, int64_t n)
+{
+    if (fn < SGPU_FN_LOG || fn > SGPU_FN_LGAMMA || n < 0 || (n > 0 && (!x || !y))) {
+        return fail(SGPU_ERR_ARG, "sgpu_eval_math: fn %d, n %lld", fn, (long long)n);
+    }
+    if (n == 0) {
+        return SGPU_OK;
+    }
+    const size_t bytes = (size_t)n * sizeof(double);
+    DevBuf dx, dy;
+    HIP_OK(hipMalloc(&dx.p, bytes));
+    HIP_OK(hipMalloc(&dy.p, bytes));
+    HIP_OK(hipMemcpy(dx.p, x, bytes, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_eval_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, fn,
+                       (const double *)dx.p, (double *)dy.p, n);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpy(y, dy.p, bytes, hipMemcpyDeviceToHost));
     return SGPU_OK;
 }
 

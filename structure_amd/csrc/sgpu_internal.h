@@ -34,6 +34,7 @@ struct ModelArgs {
     int freqscorr, onefst, popalphas, unifprioralpha, popspecificlambda;
     int inferalpha, inferlambda, pad0;
     double alphapropsd, alphamax, alphapriora, alphapriorb, fpriormean, fpriorsd;
+    double invsqrtn;   // N^-1/2 from the host's pow, the epsilon step bound (UpdateEpsilon.c:80)
     uint32_t seed, chain;
 };
 
