@@ -59,11 +59,30 @@ __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ 
     const size_t pos = (size_t)idx * A + aa;
     double x = 0.0;
     if (act) {
+        // the partials are independent loads: eight in flight, exact integer sums
         const size_t stride = (size_t)m.L * m.K * A;
-        int n = 0;
-        for (int it = 0; it < C.n_it; it++) {
-            n += nApart[it * stride + pos];
+        const int n_it = C.n_it;
+        int n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+        int it = 0;
+        for (; it + 8 <= n_it; it += 8) {
+            const SGPU_G int32_t *q = nApart + it * stride + pos;
+            n0 += q[0] + q[4 * stride];
+            n1 += q[stride] + q[5 * stride];
+            n2 += q[2 * stride] + q[6 * stride];
+            n3 += q[3 * stride] + q[7 * stride];
         }
+        if (it + 4 <= n_it) {
+            const SGPU_G int32_t *q = nApart + it * stride + pos;
+            n0 += q[0];
+            n1 += q[stride];
+            n2 += q[2 * stride];
+            n3 += q[3 * stride];
+            it += 4;
+        }
+        for (; it < n_it; it++) {
+            n0 += nApart[it * stride + pos];
+        }
+        const int n = (n0 + n1) + (n2 + n3);
         const double Fk = F[k];
         const double par = m.freqscorr ? eps[(size_t)l * A + aa] * (1.0 - Fk) / Fk + (double)n
                                        : lambda[k] + (double)n;
@@ -239,12 +258,30 @@ __global__ __launch_bounds__(kNT) void k_logq_stats(const ChainDesc *__restrict_
 
 // ---------------------------------------------------------------------------
 // UpdateQMetro (UpdateQ.c:199-307): proposal from the prior, accepted with
-// probability exp(sum over copies of log_d(TestQ.P) - log_d(Q.P))
-// (CalcLikeIndDiff, CalcLike.c:281-345).  Pass 1: one thread per
-// individual and a locus range per grid row; each thread regenerates its
-// proposal from the stream and accumulates both products in FP64 with
-// exponent bookkeeping.  Pass 2 sums the ranges and decides.
+// probability exp(sum over copies of log(TestQ.P) - log(Q.P))
+// (CalcLikeIndDiff, CalcLike.c:281-345).  Pass 0 draws each individual's
+// proposal once into qprop; pass 1 (one thread per individual and a locus
+// range per grid row) accumulates both products in FP64 with exponent
+// bookkeeping; pass 2 sums the ranges and decides.
 // ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNT) void k_metro_propose(const ChainDesc *__restrict__ cds,
+                                                        const BlockRef *__restrict__ map,
+                                                        const uint32_t *__restrict__ repp)
+{
+    SGPU_BLOCK_PROLOGUE();
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    const int i = bx * kNT + threadIdx.x;
+    if (i >= m.N) {
+        return;
+    }
+    double q[kMaxK];
+    dirichlet_q(m, g(C.alpha), nullptr, rep, i, PH_QMETRO, q);
+    SGPU_G double *qp = g(C.qprop) + (size_t)i * m.K;
+    for (int k = 0; k < m.K; k++) {
+        qp[k] = q[k];
+    }
+}
+
 constexpr int kMetroLoci = 256;   // loci of P staged in LDS at a time
 
 // Biallelic 2-bit path, K templated: the block's P columns are staged in LDS
@@ -254,32 +291,25 @@ constexpr int kMetroLoci = 256;   // loci of P staged in LDS at a time
 template <int K>
 __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__restrict__ cds,
                                                            const BlockRef *__restrict__ map,
-                                                           const uint32_t *__restrict__ repp)
+                                                           const uint32_t *__restrict__ /*repp*/)
 {
     SGPU_BLOCK_PROLOGUE();
     const int by = (int)br_.by;
-    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const SGPU_G uint32_t *geno = g(C.geno);
     const SGPU_G double *Q = g(C.Q);
     const SGPU_G double *P = g(C.P);
-    const SGPU_G double *alpha = g(C.alpha);
+    const SGPU_G double *qprop = g(C.qprop);
     SGPU_G double *delta_part = g(C.delta_part);
     const int Lpad = C.Lpad, loci_per_range = C.loci_per_range;
-    __shared__ double s_p[kMetroLoci * K * 2];
     const int i = bx * kNT + threadIdx.x;
     const bool active = i < m.N;
     double tq[K], cq[K];
-    {
-        double tmp[kMaxK];
-        if (active) {
-            dirichlet_q(m, alpha, nullptr, rep, i, PH_QMETRO, tmp);
-        }
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            tq[k] = active ? tmp[k] : 1.0;
-            cq[k] = active ? Q[(size_t)i * K + k] : 1.0;
-        }
+    for (int k = 0; k < K; k++) {
+        tq[k] = active ? qprop[(size_t)i * K + k] : 1.0;
+        cq[k] = active ? Q[(size_t)i * K + k] : 1.0;
     }
+    __shared__ double s_p[kMetroLoci * K * 2];
     const int l0 = by * loci_per_range;
     const int l1 = min(l0 + loci_per_range, m.L);
     const SGPU_G uint32_t *row = geno + (size_t)(active ? (i >> 4) : 0) * Lpad;
@@ -336,15 +366,14 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__res
 template <int K>
 __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__restrict__ cds,
                                                            const BlockRef *__restrict__ map,
-                                                           const uint32_t *__restrict__ repp)
+                                                           const uint32_t *__restrict__ /*repp*/)
 {
     SGPU_BLOCK_PROLOGUE();
     const int by = (int)br_.by;
-    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const SGPU_G uint32_t *geno = g(C.geno);
     const SGPU_G double *Q = g(C.Q);
     const SGPU_G double *P = g(C.P);
-    const SGPU_G double *alpha = g(C.alpha);
+    const SGPU_G double *qprop = g(C.qprop);
     SGPU_G double *delta_part = g(C.delta_part);
     const int Lpad = C.Lpad, loci_per_range = C.loci_per_range;
     const int i = bx * kNT + threadIdx.x;
@@ -352,14 +381,10 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__res
         return;
     }
     double tq[K], cq[K];
-    {
-        double tmp[kMaxK];
-        dirichlet_q(m, alpha, nullptr, rep, i, PH_QMETRO, tmp);
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            tq[k] = tmp[k];
-            cq[k] = Q[(size_t)i * K + k];
-        }
+    for (int k = 0; k < K; k++) {
+        tq[k] = qprop[(size_t)i * K + k];
+        cq[k] = Q[(size_t)i * K + k];
     }
     const int l0 = by * loci_per_range;
     const int l1 = min(l0 + loci_per_range, m.L);
@@ -407,7 +432,6 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restric
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const SGPU_G double *delta_part = g(C.delta_part);
-    const SGPU_G double *alpha = g(C.alpha);
     SGPU_G double *Q = g(C.Q);
     const int n_ranges = C.n_ranges;
     __shared__ double s_red[kNT / 64];
@@ -422,12 +446,9 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restric
         const u32x4 w = philox4x32_10(rep, (uint32_t)i, 0xffffffffu, PH_QMETRO << 24, m.seed,
                                       m.chain);
         const bool acc = uo(w.x) < exp(delta);
-        if (acc) {
-            dirichlet_q(m, alpha, nullptr, rep, i, PH_QMETRO, q);
-        } else {
-            for (int k = 0; k < m.K; k++) {
-                q[k] = Q[(size_t)i * m.K + k];
-            }
+        const SGPU_G double *src = acc ? g(C.qprop) : Q;
+        for (int k = 0; k < m.K; k++) {
+            q[k] = src[(size_t)i * m.K + k];
         }
     }
     finish_q<kNT>(m, bx, C.q_blocks, i, active, q, Q, g(C.QSum), collect, g(C.logq_part),
@@ -437,7 +458,7 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restric
 // ---------------------------------------------------------------------------
 // UpdateAlpha (UpdateAlpha.c:189-253), one block per chain: sums the log-Q
 // statistics and runs the pop loop in thread 0.  LogProbQ(a) = (a-1) * sum
-// log_d(clamped Q) + (lgamma(K a) - K lgamma(a)) * N; LogProbQonepop as in the
+// log(clamped Q) + (lgamma(K a) - K lgamma(a)) * N; LogProbQonepop as in the
 // reference.  As there, logprobdiff is zeroed once and only reset by the
 // gamma prior.
 // ---------------------------------------------------------------------------

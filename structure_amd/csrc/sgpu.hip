@@ -680,6 +680,7 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         (rc = dalloc(c, &d.likepart, (size_t)d.n_it * d.n_lt)) ||
         (rc = dalloc(c, &d.logq_part, cdiv(N, kGibbsInds) * (K + 1))) ||
         (rc = dalloc(c, &d.delta_part, (size_t)d.n_ranges * N)) ||
+        (rc = dalloc(c, &d.qprop, (size_t)N * K)) ||
         (rc = dalloc(c, &d.lam_part, (size_t)d.l_blocks * K)) ||
         (rc = dalloc(c, &d.fst_part, (size_t)d.l_blocks * K)) ||
         (rc = dalloc(c, &d.QSum, (size_t)N * K)) || (rc = dalloc(c, &d.PSum, (size_t)L * K * A)) ||
@@ -1125,6 +1126,9 @@ int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
     const uint32_t *r = c->repd;
     const bool metro = c->metrofreq > 0 && rep % (uint32_t)c->metrofreq == 0;
     if (metro) {
+        if ((rc = launch(c, k_metro_propose, c->p_accept, kNT, r))) {
+            return rc;
+        }
         for (const KGroup &g : c->metro_groups) {
             mkern_t k = metro_kernel(g.fmt, g.K);
             if (!k) {
