@@ -161,7 +161,6 @@ __device__ __forceinline__ void finish_q(const ModelArgs &m, int bx, int nrows, 
     }
 }
 
-constexpr int kGibbsInds = 64;     // individuals per UpdateQ block
 constexpr int kRowSplit = 16;      // second-level partials of the ancestry counts
 
 // First level of the ancestry-count reduction: the n_lt locus-tile partials of
@@ -195,35 +194,90 @@ __global__ __launch_bounds__(kNT) void k_fold_rows(const ChainDesc *__restrict__
     out[(size_t)by * n + f] = (a0 + a1) + (a2 + a3);
 }
 
-// Gibbs update of Q from the folded counts ([kRowSplit][N][K]): one thread per
-// individual, 64 per block.
-__global__ __launch_bounds__(kGibbsInds) void k_update_q_gibbs(const ChainDesc *__restrict__ cds,
-                                                               const BlockRef *__restrict__ map,
-                                                               const uint32_t *__restrict__ repp,
-                                                               int collect)
+// Gibbs update of Q (UpdateQAdmixture, UpdateQ.c:387-424; RDirichlet,
+// ran.c:348-359): one thread per (individual, pop), a block holding whole
+// individuals (K * (256 / K) threads).  Each thread sums its folded ancestry
+// counts ([kRowSplit][N][K], loads in flight) and draws one gamma; the
+// individual's sum is taken in pop order from LDS.  Fused: QSum, and the
+// log-Q statistics of the alpha update, one row per block: per pop the sum
+// over the block's individuals of log Q[i][k] (LogProbQonepop,
+// UpdateAlpha.c:102-111), and the sum of the per-individual totals of
+// log max(Q, 1e-50) (LogProbQ, UpdateAlpha.c:157-180).
+__global__ __launch_bounds__(kNT) void k_update_q_gibbs(const ChainDesc *__restrict__ cds,
+                                                         const BlockRef *__restrict__ map,
+                                                         const uint32_t *__restrict__ repp,
+                                                         int collect)
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
-    const SGPU_G int32_t *nQfold = g(C.nQfold);
     const SGPU_G double *alpha = g(C.alpha);
-    __shared__ double s_red[1];
-    const int i = bx * kGibbsInds + threadIdx.x;
-    const bool active = i < m.N;
-    double q[kMaxK];
+    SGPU_G double *Q = g(C.Q);
+    SGPU_G double *QSum = g(C.QSum);
+    SGPU_G double *logq_part = g(C.logq_part);
+    __shared__ double s_x[kNT], s_l[kNT], s_c[kNT];
+    const int K = m.K;
+    const int ipb = kNT / K;   // individuals per block
+    const int j = (int)threadIdx.x / K, k = (int)threadIdx.x - j * K;
+    const int i = bx * ipb + j;
+    const bool active = j < ipb && i < m.N;
+    double x = 0.0;
     if (active) {
-        int32_t cnt[kMaxK];
-        const size_t n = (size_t)m.N * m.K;
-        for (int k = 0; k < m.K; k++) {
-            int32_t c = 0;
-            for (int s2 = 0; s2 < kRowSplit; s2++) {
-                c += nQfold[s2 * n + (size_t)i * m.K + k];
-            }
-            cnt[k] = c;
+        const size_t n = (size_t)m.N * K;
+        const SGPU_G int32_t *f = g(C.nQfold) + (size_t)i * K + k;
+        int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+#pragma unroll
+        for (int s2 = 0; s2 < kRowSplit; s2 += 4) {
+            c0 += f[(size_t)s2 * n];
+            c1 += f[(size_t)(s2 + 1) * n];
+            c2 += f[(size_t)(s2 + 2) * n];
+            c3 += f[(size_t)(s2 + 3) * n];
         }
-        dirichlet_q(m, alpha, cnt, rep, i, PH_Q, q);
+        x = gamma_draw(alpha[k] + (double)((c0 + c1) + (c2 + c3)), m.seed, m.chain, rep,
+                       (uint32_t)i, (uint32_t)k, PH_Q);
     }
-    finish_q<kGibbsInds>(m, bx, (m.N + kGibbsInds - 1) / kGibbsInds, i, active, q, g(C.Q),
-                         g(C.QSum), collect, g(C.logq_part), g(C.logq_rows), s_red);
+    s_x[threadIdx.x] = x;
+    __syncthreads();
+    double q = 0.0;
+    if (active) {
+        double sum = 0.0;
+        for (int kk = 0; kk < K; kk++) {   // pop order, as RDirichlet adds them
+            sum += s_x[threadIdx.x - k + kk];
+        }
+        q = x / sum;
+        Q[(size_t)i * K + k] = q;
+        if (collect) {
+            QSum[(size_t)i * K + k] += q;
+        }
+    }
+    s_l[threadIdx.x] = active ? log_d(q) : 0.0;
+    s_c[threadIdx.x] = active ? log_d(q > kSqrtUnder ? q : kSqrtUnder) : 0.0;
+    __syncthreads();
+    // per individual, the clamped total in pop order (into s_x, now free) ...
+    if (k == 0 && j < ipb) {
+        double tot = 0.0;
+        for (int kk = 0; kk < K; kk++) {
+            tot += s_c[threadIdx.x + kk];
+        }
+        s_x[j] = tot;
+    }
+    __syncthreads();
+    // ... then a pairwise tree over the block's individuals, per pop and for the totals
+    for (int st = 1; st < ipb; st *= 2) {
+        if (j % (2 * st) == 0 && j + st < ipb) {
+            s_l[threadIdx.x] += s_l[threadIdx.x + st * K];
+            if (k == 0) {
+                s_x[j] += s_x[j + st];
+            }
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x <= K) {
+        logq_part[(size_t)bx * (K + 1) + threadIdx.x] =
+            (int)threadIdx.x < K ? s_l[threadIdx.x] : s_x[0];
+    }
+    if (bx == 0 && threadIdx.x == 0) {
+        g(C.logq_rows)[0] = (m.N + ipb - 1) / ipb;
+    }
 }
 
 // Log-Q statistics of the current Q (after initialisation or a host write of Q).
@@ -270,15 +324,24 @@ __global__ __launch_bounds__(kNT) void k_metro_propose(const ChainDesc *__restri
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
-    const int i = bx * kNT + threadIdx.x;
-    if (i >= m.N) {
-        return;
+    __shared__ double s_x[kNT];
+    const int K = m.K;
+    const int ipb = kNT / K;   // one thread per (individual, pop), whole individuals per block
+    const int j = (int)threadIdx.x / K, k = (int)threadIdx.x - j * K;
+    const int i = bx * ipb + j;
+    const bool active = j < ipb && i < m.N;
+    double x = 0.0;
+    if (active) {
+        x = gamma_draw(g(C.alpha)[k], m.seed, m.chain, rep, (uint32_t)i, (uint32_t)k, PH_QMETRO);
     }
-    double q[kMaxK];
-    dirichlet_q(m, g(C.alpha), nullptr, rep, i, PH_QMETRO, q);
-    SGPU_G double *qp = g(C.qprop) + (size_t)i * m.K;
-    for (int k = 0; k < m.K; k++) {
-        qp[k] = q[k];
+    s_x[threadIdx.x] = x;
+    __syncthreads();
+    if (active) {
+        double sum = 0.0;
+        for (int kk = 0; kk < K; kk++) {   // pop order, as RDirichletDisc adds them
+            sum += s_x[threadIdx.x - k + kk];
+        }
+        g(C.qprop)[(size_t)i * K + k] = x / sum;
     }
 }
 
@@ -439,10 +502,19 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restric
     const bool active = i < m.N;
     double q[kMaxK];
     if (active) {
-        double delta = 0.0;
-        for (int r = 0; r < n_ranges; r++) {
-            delta += delta_part[(size_t)r * m.N + i];
+        // the range partials in four interleaved sums (loads in flight), fixed order
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+        int r = 0;
+        for (; r + 4 <= n_ranges; r += 4) {
+            d0 += delta_part[(size_t)r * m.N + i];
+            d1 += delta_part[(size_t)(r + 1) * m.N + i];
+            d2 += delta_part[(size_t)(r + 2) * m.N + i];
+            d3 += delta_part[(size_t)(r + 3) * m.N + i];
         }
+        for (; r < n_ranges; r++) {
+            d0 += delta_part[(size_t)r * m.N + i];
+        }
+        const double delta = (d0 + d1) + (d2 + d3);
         const u32x4 w = philox4x32_10(rep, (uint32_t)i, 0xffffffffu, PH_QMETRO << 24, m.seed,
                                       m.chain);
         const bool acc = uo(w.x) < exp(delta);

@@ -678,7 +678,7 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         (rc = dalloc(c, &d.nQfold, (size_t)kRowSplit * N * K)) ||
         (rc = dalloc(c, &d.logq_rows, 1)) ||
         (rc = dalloc(c, &d.likepart, (size_t)d.n_it * d.n_lt)) ||
-        (rc = dalloc(c, &d.logq_part, cdiv(N, kGibbsInds) * (K + 1))) ||
+        (rc = dalloc(c, &d.logq_part, cdiv(N, kNT / K) * (K + 1))) ||
         (rc = dalloc(c, &d.delta_part, (size_t)d.n_ranges * N)) ||
         (rc = dalloc(c, &d.qprop, (size_t)N * K)) ||
         (rc = dalloc(c, &d.lam_part, (size_t)d.l_blocks * K)) ||
@@ -720,7 +720,7 @@ int build_plans(sgpu_ctx *c)
                              return std::make_pair((int)cdiv((size_t)h.m.N * h.m.K, kNT), kRowSplit);
                          })) ||
         (rc = build_plan(c, c->p_gibbs, all_chains,
-                         [](const HostChain &h) { return std::make_pair((int)cdiv(h.m.N, kGibbsInds), 1); })) ||
+                         [](const HostChain &h) { return std::make_pair((int)cdiv(h.m.N, kNT / h.m.K), 1); })) ||
         (rc = build_plan(c, c->p_logq, all_chains, qb)) ||
         (rc = build_plan(c, c->p_accept, all_chains, qb)) ||
         (rc = build_plan(c, c->p_alpha, [](const HostChain &h) { return h.m.inferalpha != 0; }, one)) ||
@@ -1126,7 +1126,7 @@ int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
     const uint32_t *r = c->repd;
     const bool metro = c->metrofreq > 0 && rep % (uint32_t)c->metrofreq == 0;
     if (metro) {
-        if ((rc = launch(c, k_metro_propose, c->p_accept, kNT, r))) {
+        if ((rc = launch(c, k_metro_propose, c->p_gibbs, kNT, r))) {   // (individual, pop) grid
             return rc;
         }
         for (const KGroup &g : c->metro_groups) {
@@ -1143,7 +1143,7 @@ int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
     if ((rc = launch(c, k_fold_rows, c->p_fold, kNT))) {
         return rc;
     }
-    return launch(c, k_update_q_gibbs, c->p_gibbs, kGibbsInds, r, (int)collect);
+    return launch(c, k_update_q_gibbs, c->p_gibbs, kNT, r, (int)collect);
 }
 
 int sgpu_update_z(sgpu_ctx *c, uint32_t rep, int32_t flags, int8_t *zout)
