@@ -25,6 +25,31 @@ constexpr int kNT = 256;   // threads per block for the update kernels
     const int bx = (int)br_.bx;                        \
     (void)bx
 
+// Sum over b < n of x[b * stride + off] by the 64 lanes of one wave: lane j
+// takes b = j, j + 64, ... into eight interleaved accumulators (eight loads in
+// flight), then a fixed tree over the accumulators and the lanes; the same
+// order whichever wave of a block runs it.  Valid in lane 0.
+__device__ __forceinline__ double wave_strided_sum(const SGPU_G double *x, int n, int stride,
+                                                   int off)
+{
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int b = (int)(threadIdx.x & 63);
+    for (; b + 7 * 64 < n; b += 8 * 64) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            a[u] += x[(size_t)(b + u * 64) * stride + off];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        if (b + u * 64 < n) {
+            a[u] += x[(size_t)(b + u * 64) * stride + off];
+        }
+    }
+    const double v = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    return wave_sum(v);
+}
+
 // ---------------------------------------------------------------------------
 // UpdateP (UpdateP.c:106-199): one thread per (locus, pop, allele), a block
 // holding whole (locus, pop) groups (A * (256 / A) threads).  Dirichlet
@@ -528,34 +553,25 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restric
 }
 
 // ---------------------------------------------------------------------------
-// UpdateAlpha (UpdateAlpha.c:189-253), one block per chain: sums the log-Q
-// statistics and runs the pop loop in thread 0.  LogProbQ(a) = (a-1) * sum
+// UpdateAlpha (UpdateAlpha.c:189-253), one wave per chain: sums the log-Q
+// statistics and runs the pop loop in lane 0.  LogProbQ(a) = (a-1) * sum
 // log(clamped Q) + (lgamma(K a) - K lgamma(a)) * N; LogProbQonepop as in the
 // reference.  As there, logprobdiff is zeroed once and only reset by the
 // gamma prior.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNT) void k_update_alpha(const ChainDesc *__restrict__ cds,
-                                                       const BlockRef *__restrict__ map,
-                                                       const uint32_t *__restrict__ repp)
+__device__ void alpha_wave(const ChainDesc &C, const ModelArgs &m, uint32_t rep)
 {
-    SGPU_BLOCK_PROLOGUE();
-    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const SGPU_G double *logq_part = g(C.logq_part);
     SGPU_G double *alpha = g(C.alpha);
     const int nparts = g(C.logq_rows)[0];
-    __shared__ double s_red[kNT / 64];
-    __shared__ double s_stat[kMaxK + 1];
+    __shared__ double s_stat[kMaxK + 1];   // written and read by lane 0 only
     for (int k = 0; k <= m.K; k++) {
-        double v = 0.0;
-        for (int b = threadIdx.x; b < nparts; b += kNT) {
-            v += logq_part[(size_t)b * (m.K + 1) + k];
-        }
-        const double s = block_sum<kNT>(v, s_red);
-        if (threadIdx.x == 0) {
-            s_stat[k] = s;
+        const double t = wave_strided_sum(logq_part, nparts, m.K + 1, k);
+        if ((threadIdx.x & 63) == 0) {
+            s_stat[k] = t;
         }
     }
-    if (threadIdx.x != 0) {
+    if ((threadIdx.x & 63) != 0) {
         return;
     }
     const int K = m.K;
@@ -599,6 +615,16 @@ __global__ __launch_bounds__(kNT) void k_update_alpha(const ChainDesc *__restric
                 }
             }
         }
+    }
+}
+
+__global__ __launch_bounds__(kNT) void k_update_alpha(const ChainDesc *__restrict__ cds,
+                                                       const BlockRef *__restrict__ map,
+                                                       const uint32_t *__restrict__ repp)
+{
+    SGPU_BLOCK_PROLOGUE();
+    if (threadIdx.x < 64) {
+        alpha_wave(C, m, *repp);
     }
 }
 
@@ -847,32 +873,20 @@ __global__ __launch_bounds__(kNT) void k_fst_partial(const ChainDesc *__restrict
     }
 }
 
-__global__ __launch_bounds__(kNT) void k_fst_finalize(const ChainDesc *__restrict__ cds,
-                                                       const BlockRef *__restrict__ map,
-                                                       const uint32_t *__restrict__ repp)
+__device__ void fst_final_wave(const ChainDesc &C, const ModelArgs &m, uint32_t rep)
 {
-    SGPU_BLOCK_PROLOGUE();
-    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const SGPU_G double *part = g(C.fst_part);
     SGPU_G double *F = g(C.F);
     const int nparts = C.l_blocks, nloci_nonzero = C.nloci_nonzero;
-    __shared__ double s_red[kNT / 64];
-    __shared__ double s_stat[kMaxK];
     const int numpops1 = m.onefst ? 1 : m.K;
-    for (int p1 = 0; p1 < numpops1; p1++) {
-        double v = 0.0;
-        for (int b = threadIdx.x; b < nparts; b += kNT) {
-            v += part[(size_t)b * numpops1 + p1];
-        }
-        const double t = block_sum<kNT>(v, s_red);
-        if (threadIdx.x == 0) {
-            s_stat[p1] = t;
-        }
-    }
-    __syncthreads();
     // the proposal groups are independent (each reads and writes its own F), so
-    // thread p1 decides group p1
-    const int p1 = threadIdx.x;
+    // lane p1 decides group p1 with the group's sum of block partials
+    const int p1 = (int)(threadIdx.x & 63);
+    double stat = 0.0;
+    for (int q = 0; q < numpops1; q++) {
+        const double t = __shfl(wave_strided_sum(part, nparts, numpops1, q), 0, 64);
+        stat = p1 == q ? t : stat;
+    }
     if (p1 >= numpops1) {
         return;
     }
@@ -886,7 +900,7 @@ __global__ __launch_bounds__(kNT) void k_fst_finalize(const ChainDesc *__restric
         const double ofr = (1.0 - oldf) / oldf;
         const int ngroup = m.onefst ? m.K : 1;
         lpd += (double)ngroup * (double)nloci_nonzero * (lgamma_lanczos(nfr) - lgamma_lanczos(ofr));
-        lpd += s_stat[p1];
+        lpd += stat;
         const bool acc = (lpd >= 0.0) || (uo(uw) < exp(lpd));
         if (acc) {
             const int p2end = m.onefst ? m.K : p1 + 1;
@@ -897,27 +911,30 @@ __global__ __launch_bounds__(kNT) void k_fst_finalize(const ChainDesc *__restric
     }
 }
 
+__global__ __launch_bounds__(kNT) void k_fst_finalize(const ChainDesc *__restrict__ cds,
+                                                       const BlockRef *__restrict__ map,
+                                                       const uint32_t *__restrict__ repp)
+{
+    SGPU_BLOCK_PROLOGUE();
+    if (threadIdx.x < 64) {
+        fst_final_wave(C, m, *repp);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Data-collection tail (output.c:50-101, UpdateSums :654-707): scalar sums
 // and lnP(D) from the UpdateZ partials (one block per chain, fixed order).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNT) void k_collect(const ChainDesc *__restrict__ cds,
-                                                  const BlockRef *__restrict__ map, int with_like)
+// lnP(D) of the last UpdateZ: the sum of its tile partials by one wave (lane 0).
+__device__ __forceinline__ double like_wave(const ChainDesc &C)
 {
-    SGPU_BLOCK_PROLOGUE();
-    const SGPU_G double *likepart = g(C.likepart);
-    const int nlike = C.n_it * C.n_lt;
-    __shared__ double s_red[kNT / 64];
-    double v = 0.0;
-    if (with_like) {
-        for (int b = threadIdx.x; b < nlike; b += kNT) {
-            v += likepart[b];
-        }
-    }
-    const double like = block_sum<kNT>(v, s_red);
-    if (threadIdx.x != 0) {
-        return;
-    }
+    return wave_strided_sum(g(C.likepart), C.n_it * C.n_lt, 1, 0);
+}
+
+// The scalar posterior sums and the lnP accumulators, one thread.
+__device__ __forceinline__ void collect_scalars(const ChainDesc &C, const ModelArgs &m,
+                                                bool with_like, double like)
+{
     for (int k = 0; k < m.K; k++) {
         g(C.lambdaSum)[k] += g(C.lambda)[k];
         g(C.alphaSum)[k] += g(C.alpha)[k];
@@ -930,6 +947,62 @@ __global__ __launch_bounds__(kNT) void k_collect(const ChainDesc *__restrict__ c
         like3[0] = like;
         like3[1] += like;
         like3[2] += like * like;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_collect(const ChainDesc *__restrict__ cds,
+                                                const BlockRef *__restrict__ map, int with_like)
+{
+    SGPU_BLOCK_PROLOGUE();
+    const double like = with_like ? like_wave(C) : 0.0;
+    if (threadIdx.x == 0) {
+        collect_scalars(C, m, with_like != 0, like);
+    }
+}
+
+// The single-block passes that close a sweep, one launch (one block per chain,
+// one wave each): alpha (it reads only Q and alpha, so it may follow Z, lambda,
+// epsilon and Fst), the Fst decision, and the data collection.  With kTailAdvance the block that
+// finishes last (counted in *done, which it resets) advances the device sweep
+// index: every block has used the index by then, so a graph replay may end here.
+constexpr int kTailAlpha = 1, kTailFst = 2, kTailCollect = 4, kTailLike = 8, kTailAdvance = 16;
+
+__global__ __launch_bounds__(kNT) void k_sweep_tail(const ChainDesc *__restrict__ cds,
+                                                     const BlockRef *__restrict__ map,
+                                                     uint32_t *__restrict__ repd,
+                                                     uint32_t *__restrict__ done, int flags)
+{
+    SGPU_BLOCK_PROLOGUE();
+    const uint32_t rep = *repd;
+    const int wid = (int)(threadIdx.x >> 6);
+    __shared__ double s_like;
+    // three independent waves: alpha, the Fst decision, the lnP sum
+    if (wid == 0 && (flags & kTailAlpha) && m.inferalpha) {
+        alpha_wave(C, m, rep);
+    }
+    if (wid == 1 && (flags & kTailFst) && m.freqscorr) {
+        fst_final_wave(C, m, rep);
+    }
+    if (wid == 2 && (flags & kTailLike)) {
+        const double like = like_wave(C);
+        if ((threadIdx.x & 63) == 0) {
+            s_like = like;
+        }
+    }
+    __syncthreads();   // alpha, F and the lnP sum are read by thread 0 below
+    if ((flags & kTailCollect) && threadIdx.x == 0) {
+        collect_scalars(C, m, (flags & kTailLike) != 0, s_like);
+    }
+    if (flags & kTailAdvance) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t prev =
+                __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev + 1u == gridDim.x) {
+                __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(repd, rep + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
     }
 }
 

@@ -123,6 +123,7 @@ struct sgpu_ctx {
     int64_t nsum = 0;
     bool initialized = false;
     bool like_ready = false;
+    bool tail_advances = false;       // the last sweep's tail launch advances *repd
     int8_t *zdbg = nullptr;
     std::vector<void *> allocs;
     // phase timing (sgpu_set_timing)
@@ -924,8 +925,12 @@ int sgpu_create_batch(const sgpu_model *models, int32_t nchains, const sgpu_data
         (rc = dalloc(c, &c->repd, 2)) || (rc = upload_descriptors(c)) || (rc = build_plans(c))) {
         return cleanup(rc);
     }
-    hipError_t e = hipMemcpyAsync(c->lambda0, lam0.data(), nchains * sizeof(double),
-                                  hipMemcpyHostToDevice, c->stream);
+    // repd[0]: the device sweep index; repd[1]: the tail pass's completion count
+    hipError_t e = hipMemsetAsync(c->repd, 0, 2 * sizeof(uint32_t), c->stream);
+    if (e == hipSuccess) {
+        e = hipMemcpyAsync(c->lambda0, lam0.data(), nchains * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream);
+    }
     if (e == hipSuccess) {
         e = hipStreamSynchronize(c->stream);
     }
@@ -1236,7 +1241,7 @@ int sgpu_collect(sgpu_ctx *c, uint32_t rep)
     }
     PhaseTimer timer(c, SGPU_PH_COLLECT);
     const int with_like = (c->computeprob && c->like_ready) ? 1 : 0;
-    if ((rc = launch(c, k_collect, c->p_collect, kNT, with_like))) {
+    if ((rc = launch(c, k_collect, c->p_collect, 64, with_like))) {
         return rc;
     }
     c->nsum++;
@@ -1254,13 +1259,35 @@ int sgpu_sweep(sgpu_ctx *c, uint32_t rep)
     if ((rc = sgpu_update_q(c, rep, collect))) return rc;
     if ((rc = sgpu_update_z(c, rep, (collect && c->computeprob) ? SGPU_Z_LIKE : 0, nullptr)))
         return rc;
-    if (c->any_alpha && (rc = sgpu_update_alpha(c, rep))) return rc;
     if (c->any_lambda && (rc = sgpu_update_lambda(c, rep))) return rc;
     if (c->any_freqscorr) {
         if ((rc = sgpu_update_eps(c, rep, collect))) return rc;
-        if ((rc = sgpu_update_fst(c, rep))) return rc;
+        PhaseTimer timer(c, SGPU_PH_FST);
+        if ((rc = launch(c, k_fst_partial, c->p_fst_part, kNT, (const uint32_t *)c->repd))) {
+            return rc;
+        }
     }
-    if (collect && (rc = sgpu_collect(c, rep))) return rc;
+    // alpha, the Fst decision and the data collection in one single-block-per-chain
+    // launch (alpha reads only Q and alpha, so it may close the sweep); in a
+    // captured sweep the same launch advances the device sweep index
+    int flags = (c->any_alpha ? kTailAlpha : 0) | (c->any_freqscorr ? kTailFst : 0);
+    if (collect) {
+        flags |= kTailCollect | ((c->computeprob && c->like_ready) ? kTailLike : 0);
+    }
+    c->tail_advances = false;
+    if (flags) {
+        if (c->capturing) {
+            flags |= kTailAdvance;
+        }
+        PhaseTimer timer(c, SGPU_PH_COLLECT);
+        if ((rc = launch(c, k_sweep_tail, c->p_collect, kNT, c->repd, c->repd + 1, flags))) {
+            return rc;
+        }
+        c->tail_advances = (flags & kTailAdvance) != 0;
+        if (collect) {
+            c->nsum++;
+        }
+    }
     return SGPU_OK;
 }
 
@@ -1293,7 +1320,7 @@ int sgpu_run_sweeps(sgpu_ctx *c, uint32_t rep0, uint32_t n)
             HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             c->capturing = true;
             rc = sgpu_sweep(c, rep);
-            if (rc == SGPU_OK) {
+            if (rc == SGPU_OK && !c->tail_advances) {
                 hipLaunchKernelGGL(k_advance_rep, dim3(1), dim3(1), 0, c->stream, c->repd);
             }
             c->capturing = false;
