@@ -22,6 +22,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+VALU_SIMDS = 1024          # MI355X: 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9           # MI355X peak engine clock
 
 
 def z_bytes_per_launch(N, L, K, A):
@@ -366,14 +368,24 @@ def main():
     z_avg_s = zms / max(zn, 1) / 1e3
     zb = z_bytes_per_launch(N, L, K, 2)
     achieved = zb / z_avg_s / 1e9
-    traffic = None
+    traffic, valu_instr = None, None
     tpath = os.path.join(ROOT, "profiles", "updatez_c2_traffic.json")
     if os.path.exists(tpath):
         try:
             with open(tpath) as f:
-                traffic = json.load(f).get("bytes_per_launch")
+                prof = json.load(f)
+            traffic = prof.get("bytes_per_launch")
+            valu_instr = prof.get("valu_instr_per_launch")
         except (OSError, ValueError):
-            traffic = None
+            traffic, valu_instr = None, None
+    # the binding bound: VALU issue, one wave64 instruction per 4 cycles per SIMD
+    valu = None
+    if valu_instr:
+        peak_ips = VALU_SIMDS * CLOCK_HZ / 4.0
+        valu = {"instr_per_launch": valu_instr, "source": "rocprofv3 SQ_INSTS_VALU of this kernel",
+                "achieved_instr_per_s": round(valu_instr / z_avg_s, 1),
+                "peak_instr_per_s": peak_ips,
+                "issue_frac": round(valu_instr / z_avg_s / peak_ips, 4)}
     result = {
         "metric": "sweeps/s at N=2k,L=100k,K=5 (%HBM peak); batched vs plain chain-sweeps/s",
         "value": round(sweeps_per_s, 3),
@@ -394,7 +406,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
                      "traffic": traffic, "kernel": "k_update_z_snp2 (fused UpdateZ + counts + lnP)",
                      "algorithmic_bytes_per_launch": zb, "avg_launch_ms": round(z_avg_s * 1e3, 4),
-                     "launches_timed": zn},
+                     "launches_timed": zn, "valu": valu},
         "site_updates_per_s": round(N * L * K * sweeps_per_s, 1),
         "sweep_hbm_frac": round(sweep_bytes(N, L, K, 2) * sweeps_per_s / 1e9 / PEAK_HBM_GBS, 5),
         "phase_ms_per_sweep": {p: round(v[0] / nt, 4) for p, v in timing.items()},

@@ -730,6 +730,17 @@ __global__ __launch_bounds__(kNT) void k_lambda_finalize(const ChainDesc *__rest
     }
 }
 
+// Fst proposal of group pop1 (UpdateFst.c:160-166): F + FPRIORSD * N(0,1).
+__device__ __forceinline__ double fst_proposal(const ModelArgs &m, double oldf, int pop1,
+                                               uint32_t rep, uint32_t *uw)
+{
+    const u32x4 w = philox4x32_10(rep, (uint32_t)pop1, 0u, PH_FST << 24, m.seed, m.chain);
+    if (uw) {
+        *uw = w.z;
+    }
+    return oldf + m.fpriorsd * box_muller(w.x, w.y);
+}
+
 // ---------------------------------------------------------------------------
 // Epsilon (NonIndependenceUpdateEpsilon, UpdateEpsilon.c:71-145).  One thread
 // per (locus, pop) term; a block holds whole loci (K * (256 / K) threads) so
@@ -772,6 +783,10 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
         e2 = e[a2];
         inrange = (e1 + d < 1.0) && (e2 - d > 0.0);
     }
+    if (bx == 0 && (int)threadIdx.x < (m.onefst ? 1 : m.K)) {
+        // this sweep's Fst proposals (F does not change before the Fst update)
+        g(C.fprop)[threadIdx.x] = fst_proposal(m, F[threadIdx.x], threadIdx.x, rep, nullptr);
+    }
     if (inrange) {
         const double frac = (1.0 - F[k]) / F[k];
         s_term[0][threadIdx.x] = lgamma_lanczos(frac * e1);
@@ -796,7 +811,9 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
             }
             const double lam = lambda[0];
             if (lam != 1.0) {
-                sum += log_d(pow((e1 + d) * (e2 - d) / (e1) / (e2), (double)lam - 1.0));
+                // log(pow(x, lam - 1)) of UpdateEpsilon.c:117-118 as (lam - 1) log x
+                // (equal but for rounding, and no pow)
+                sum += ((double)lam - 1.0) * log_d((e1 + d) * (e2 - d) / (e1) / (e2));
             }
             if (uo(w.w) < exp(sum)) {
                 e[a1] = e1 + d;
@@ -813,25 +830,17 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
 
 // ---------------------------------------------------------------------------
 // Fst (UpdateFst.c:142-204).  Pass 1 (grid: locus blocks x proposal groups)
-// draws its group's proposal once per block and sums, per locus, the allele terms of
-// FlikeFreqs(new) - FlikeFreqs(old) (UpdateFst.c:22-51); pass 2 (one block
-// per chain) adds the prior ratio (FPriorDiff, :10-18) and the per-locus
-// lgamma(frac) terms and decides each group in its own thread.  ONEFST: one
+// takes its group's proposal (drawn by the epsilon pass of the sweep, or once
+// per block when called alone) and sums, per locus, the allele terms of
+// FlikeFreqs(new) - FlikeFreqs(old) (UpdateFst.c:22-51); pass 2 (one wave per
+// chain) adds the prior ratio (FPriorDiff, :10-18) and the per-locus
+// lgamma(frac) terms and decides each group in its own lane.  ONEFST: one
 // proposal, likelihood summed over all pops.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double fst_proposal(const ModelArgs &m, double oldf, int pop1,
-                                               uint32_t rep, uint32_t *uw)
-{
-    const u32x4 w = philox4x32_10(rep, (uint32_t)pop1, 0u, PH_FST << 24, m.seed, m.chain);
-    if (uw) {
-        *uw = w.z;
-    }
-    return oldf + m.fpriorsd * box_muller(w.x, w.y);
-}
-
 __global__ __launch_bounds__(kNT) void k_fst_partial(const ChainDesc *__restrict__ cds,
                                                       const BlockRef *__restrict__ map,
-                                                      const uint32_t *__restrict__ repp)
+                                                      const uint32_t *__restrict__ repp,
+                                                      int use_fprop)
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
@@ -844,15 +853,21 @@ __global__ __launch_bounds__(kNT) void k_fst_partial(const ChainDesc *__restrict
     __shared__ double s_f[2];
     const int numpops1 = m.onefst ? 1 : m.K;
     const int p1 = (int)br_.by;   // the proposal group of this block
-    if (threadIdx.x == 0) {       // the group's proposal, drawn once per block
-        s_f[0] = F[p1];
-        s_f[1] = fst_proposal(m, s_f[0], p1, rep, nullptr);
+    double oldf, newf;
+    if (use_fprop) {              // drawn by the epsilon pass of this sweep
+        oldf = F[p1];
+        newf = g(C.fprop)[p1];
+    } else {                      // drawn here, once per block
+        if (threadIdx.x == 0) {
+            s_f[0] = F[p1];
+            s_f[1] = fst_proposal(m, s_f[0], p1, rep, nullptr);
+        }
+        __syncthreads();
+        oldf = s_f[0];
+        newf = s_f[1];
     }
-    __syncthreads();
     const int l = bx * kNT + threadIdx.x;
     const int na = (l < m.L) ? nall[l] : 0;
-    const double oldf = s_f[0];
-    const double newf = s_f[1];
     double v = 0.0;
     if (newf > 0.0 && newf < 1.0 && na > 0) {
         const double nfr = (1.0 - newf) / newf;

@@ -681,7 +681,7 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         (rc = dalloc(c, &d.likepart, (size_t)d.n_it * d.n_lt)) ||
         (rc = dalloc(c, &d.logq_part, cdiv(N, kNT / K) * (K + 1))) ||
         (rc = dalloc(c, &d.delta_part, (size_t)d.n_ranges * N)) ||
-        (rc = dalloc(c, &d.qprop, (size_t)N * K)) ||
+        (rc = dalloc(c, &d.qprop, (size_t)N * K)) || (rc = dalloc(c, &d.fprop, (size_t)K)) ||
         (rc = dalloc(c, &d.lam_part, (size_t)d.l_blocks * K)) ||
         (rc = dalloc(c, &d.fst_part, (size_t)d.l_blocks * K)) ||
         (rc = dalloc(c, &d.QSum, (size_t)N * K)) || (rc = dalloc(c, &d.PSum, (size_t)L * K * A)) ||
@@ -1226,7 +1226,7 @@ int sgpu_update_fst(sgpu_ctx *c, uint32_t rep)
     }
     PhaseTimer timer(c, SGPU_PH_FST);
     const uint32_t *r = c->repd;
-    if ((rc = launch(c, k_fst_partial, c->p_fst_part, kNT, r))) {
+    if ((rc = launch(c, k_fst_partial, c->p_fst_part, kNT, r, 0))) {
         return rc;
     }
     return launch(c, k_fst_finalize, c->p_fst_fin, kNT, r);
@@ -1263,7 +1263,7 @@ int sgpu_sweep(sgpu_ctx *c, uint32_t rep)
     if (c->any_freqscorr) {
         if ((rc = sgpu_update_eps(c, rep, collect))) return rc;
         PhaseTimer timer(c, SGPU_PH_FST);
-        if ((rc = launch(c, k_fst_partial, c->p_fst_part, kNT, (const uint32_t *)c->repd))) {
+        if ((rc = launch(c, k_fst_partial, c->p_fst_part, kNT, (const uint32_t *)c->repd, 1))) {
             return rc;
         }
     }

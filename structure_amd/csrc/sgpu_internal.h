@@ -62,6 +62,7 @@ struct ChainDesc {
     double *logq_part;             // [rows][K+1]
     double *delta_part;            // [n_ranges][N]
     double *qprop;                 // [N][K] Metropolis proposals of the current sweep
+    double *fprop;                 // [K] Fst proposals of the current sweep (epsilon pass)
     double *lam_part, *fst_part;   // [l_blocks][K]
     double *QSum, *PSum, *epsSum, *FstSum, *alphaSum, *lambdaSum, *like3;
     int8_t *zout;                  // debug copy of Z [N][2][L] or null
