@@ -165,7 +165,7 @@ def workload_c3(args):
            "phase_ms_per_sweep": {p: round(v[0] / max(min(steps, 5), 1), 4)
                                   for p, v in timing.items()}}
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(N, L, K, 500, 3, args.seed, ktrue=8, flo=0.005,
+        res["cpu_baseline"] = cpu_baseline(N, L, K, 2000, 5, args.seed, ktrue=8, flo=0.005,
                                            fhi=0.05, qalpha=0.2)
     return res
 
@@ -210,6 +210,14 @@ def _batch_vs_plain(models, datasets, doc, warm, sweeps, plain_sweeps=None):
     import structure_amd as sa
     b = sa.Batch(models, datasets, doc)
     t_batch = _timed_run(b, warm, sweeps)
+    # per-phase device time of a batch sweep (events around each phase, direct launches)
+    nt = min(sweeps, 20)
+    b.set_timing(True)
+    b.get_timing()
+    b.run(warm + sweeps, nt)
+    tm = b.get_timing()
+    b.set_timing(False)
+    phases = {p: round(v[0] / nt, 4) for p, v in tm.items()}
     b.close()
     ps = plain_sweeps or sweeps
     t_plain = 0.0
@@ -219,7 +227,7 @@ def _batch_vs_plain(models, datasets, doc, warm, sweeps, plain_sweeps=None):
         t_plain += _timed_run(ch, warm, ps)
         ch.close()
     n = len(models)
-    return n * sweeps / t_batch, n * ps / t_plain
+    return n * sweeps / t_batch, n * ps / t_plain, phases
 
 
 def workload_c5(args):
@@ -235,7 +243,8 @@ def workload_c5(args):
                             chain=j)
               for j, (sp, (_, _, _, A)) in enumerate(zip(specs, data))]
     dsets = [(f, w, na) for f, w, na, _ in data]
-    batched, plain = _batch_vs_plain(models, dsets, list(range(len(models))), warm, sweeps)
+    batched, plain, phases = _batch_vs_plain(models, dsets, list(range(len(models))), warm,
+                                             sweeps)
     sites = sum(sp["N"] * sp["L"] * sp["K"] for sp in specs)
     return {"metric": "chain-sweeps/s, 64 ragged data sets in one batch",
             "value": round(batched, 3), "unit": "chain-sweeps/s", "n_gpus": 1, "steps": sweeps,
@@ -246,7 +255,8 @@ def workload_c5(args):
                                    "(2-20 alleles), K in [2,8], F model + alpha",
                        "datasets": len(specs), "total_sites": sum(sp["N"] * sp["L"] for sp in specs)},
             "plain_chain_sweeps_per_s": round(plain, 3), "speedup": round(batched / plain, 3),
-            "site_updates_per_s": round(sites * batched / len(specs), 1)}
+            "site_updates_per_s": round(sites * batched / len(specs), 1),
+            "phase_ms_per_batch_sweep": phases}
 
 
 def workload_c1batch(args):
@@ -261,15 +271,16 @@ def workload_c1batch(args):
     models = [sa.make_model(200, 500, 3, A, freqscorr=0, inferalpha=1, lambda0=1.0,
                             metrofreq=10, burnin=warm, numreps=sweeps, seed=args.seed + r,
                             chain=3) for r in range(100)]
-    batched, plain = _batch_vs_plain(models, [(fmt, words, d["nall"])], [0] * 100, warm,
-                                     sweeps, plain_sweeps=max(sweeps // 10, 20))
+    batched, plain, phases = _batch_vs_plain(models, [(fmt, words, d["nall"])], [0] * 100,
+                                             warm, sweeps, plain_sweeps=max(sweeps // 10, 20))
     res = {"metric": "chain-sweeps/s, 100 replicate chains of config 1 in one batch",
            "value": round(batched, 3), "unit": "chain-sweeps/s", "n_gpus": 1, "steps": sweeps,
            "warmup": warm, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": f"synthetic microsatellites (seed {args.seed})",
            "config": {"workload": "config 1 x 100 seeds: N=200 L=500 A in [2,10] K=3, "
                                   "no F model, alpha inferred", "chains": 100},
-           "plain_chain_sweeps_per_s": round(plain, 3), "speedup": round(batched / plain, 3)}
+           "plain_chain_sweeps_per_s": round(plain, 3), "speedup": round(batched / plain, 3),
+           "phase_ms_per_batch_sweep": phases}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_c1(d, A, args.seed)
     return res
@@ -283,7 +294,7 @@ def cpu_baseline_c1(d, A, seed):
     ch = oracle.Chain(m, d["geno"], d["nall"])
     ch.init()
     ch.sweep(0)
-    n = 200
+    n = 2500   # about 10 s of one core
     t0 = time.perf_counter()
     for r in range(1, 1 + n):
         ch.sweep(r)

@@ -257,6 +257,25 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Block sum over nw waves (blockDim.x = 64 nw): the same tree as block_sum<64 nw>.
+__device__ __forceinline__ double block_sum_waves(double v, double *lds /* nw */, int nw)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        lds[wid] = v;
+    }
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < nw; i++) {
+            t += lds[i];
+        }
+    }
+    __syncthreads();
+    return t;   // valid in thread 0
+}
+
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double *lds /* NT/64 */)
 {

@@ -317,7 +317,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
 // copies of individual 2h, words 2,3 those of 2h+1, the j-th non-missing copy
 // taking the j-th word of its individual (randGen.c:23-36).
 // ---------------------------------------------------------------------------
-constexpr int kByteRows = 256;   // word rows (individual pairs) per UpdateZ block
+constexpr int kByteRows = 256;   // most word rows (individual pairs) per UpdateZ block; a batch
+                                 // may use 128 or 64 (the block size)
 
 template <int K, bool INIT, bool LIKE>
 __device__ __forceinline__ void byte_copy(int x, bool valid, uint32_t w, const double *q,
@@ -383,7 +384,8 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
     double *s_p = s_dyn;                                  // [chunk][K][A] frequencies
     uint32_t *s_h = (uint32_t *)(s_dyn + (size_t)chunk * K * A);   // [chunk][K][A] counts
 
-    const int h = by * kByteRows + threadIdx.x;   // word row of this lane
+    const int R = (int)blockDim.x;                // word rows per block (64, 128 or 256)
+    const int h = by * R + threadIdx.x;           // word row of this lane
     const bool rowok = h < G;
     const int i0 = 2 * h, i1 = 2 * h + 1;
     const bool ok0 = rowok && i0 < N, ok1 = rowok && i1 < N;
@@ -406,7 +408,7 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
     for (int c0 = lt0; c0 < lt1; c0 += chunk) {
         const int nc = min(chunk, lt1 - c0);
         __syncthreads();
-        for (int t = threadIdx.x; t < nc * KA; t += kByteRows) {
+        for (int t = threadIdx.x; t < nc * KA; t += R) {
             if (!INIT) {
                 s_p[t] = P[(size_t)c0 * KA + t];
             }
@@ -459,7 +461,7 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
         __syncthreads();
         // this block alone covers (individual tile by, loci c0..c0+nc): plain stores
         SGPU_G int32_t *dst = nApart + ((size_t)by * L + c0) * KA;
-        for (int t = threadIdx.x; t < nc * KA; t += kByteRows) {
+        for (int t = threadIdx.x; t < nc * KA; t += R) {
             dst[t] = (int32_t)s_h[t];
         }
     }
@@ -487,7 +489,7 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
     if (LIKE) {
         __shared__ double s_red[kByteRows / 64];
         const double v = (ok0 || ok1) ? acc.log_value() : 0.0;
-        const double tot = block_sum<kByteRows>(v, s_red);
+        const double tot = block_sum_waves(v, s_red, R >> 6);
         if (threadIdx.x == 0) {
             likepart[(size_t)by * n_lt + bx] = tot;
         }

@@ -124,6 +124,7 @@ struct sgpu_ctx {
     bool initialized = false;
     bool like_ready = false;
     bool tail_advances = false;       // the last sweep's tail launch advances *repd
+    int byte_rows = kByteRows;        // block size of the byte-format UpdateZ launches
     int8_t *zdbg = nullptr;
     std::vector<void *> allocs;
     // phase timing (sgpu_set_timing)
@@ -298,7 +299,7 @@ int launch_z(sgpu_ctx *c, int mode)
         if (!k) {
             return fail(SGPU_ERR_UNSUPPORTED, "no UpdateZ kernel for K=%d", g.K);
         }
-        const int threads = g.fmt == SGPU_GENO_BYTE ? kByteRows : kZLoci;
+        const int threads = g.fmt == SGPU_GENO_BYTE ? c->byte_rows : kZLoci;
         hipLaunchKernelGGL(k, dim3(g.plan.nblocks), dim3(threads), g.plan.lds, c->stream, c->cds,
                            g.plan.map, (const uint32_t *)c->repd);
         LAUNCH_OK();
@@ -590,8 +591,10 @@ int upload_dataset(sgpu_ctx *c, Dataset &ds, const sgpu_data *d)
 struct BatchTiles {
     int snp_rows;   // 16-individual word rows per 2-bit UpdateZ block (before the LDS cap)
     int metro_lpr;  // loci per Metropolis block (multiple of 4)
-    int byte_loci;  // loci per byte-format UpdateZ block (0 = by data-set size)
+    int byte_loci;  // loci per byte-format UpdateZ block (0 = byte_zt)
     int byte_chunk; // loci staged in LDS at once by the byte path (0 = as many as fit)
+    int byte_rows;  // word rows (individual pairs) per byte-format UpdateZ block: 64, 128, 256
+    int byte_zt;    // default loci per byte-format UpdateZ block
 };
 
 BatchTiles batch_tiles(const sgpu_model *models, const sgpu_data *data,
@@ -608,6 +611,55 @@ BatchTiles batch_tiles(const sgpu_model *models, const sgpu_data *data,
     }
     BatchTiles t{};
     t.snp_rows = (int)std::max<long long>(1, snp_work / std::max(z_target, 1));
+    // byte format: the largest block (word rows) that launches at most 15% more
+    // lanes than 64-row blocks would, then the longest locus tile that still
+    // gives z_target blocks; failing that, the shortest tile whose per-tile
+    // ancestry-count partials (N K int32 per tile) stay within half of the
+    // genotype stream (2 N L bytes), i.e. at least 4 K loci
+    auto lanes = [&](int R) {
+        long long s = 0;
+        for (int ci = 0; ci < nchains; ci++) {
+            if (data[data_of_chain[ci]].format == SGPU_GENO_BYTE) {
+                s += (long long)cdiv(cdiv(models[ci].N, 2), R) * R * models[ci].L;
+            }
+        }
+        return s;
+    };
+    t.byte_rows = 64;
+    for (int R : {256, 128}) {
+        if (lanes(R) * 100 <= lanes(64) * 115) {
+            t.byte_rows = R;
+            break;
+        }
+    }
+    auto blocks = [&](int zt) {
+        long long s = 0;
+        for (int ci = 0; ci < nchains; ci++) {
+            if (data[data_of_chain[ci]].format == SGPU_GENO_BYTE) {
+                s += (long long)cdiv(models[ci].L, zt) * cdiv(cdiv(models[ci].N, 2), t.byte_rows);
+            }
+        }
+        return s;
+    };
+    int kmax = 1;
+    for (int ci = 0; ci < nchains; ci++) {
+        if (data[data_of_chain[ci]].format == SGPU_GENO_BYTE) {
+            kmax = std::max(kmax, models[ci].K);
+        }
+    }
+    t.byte_zt = 0;
+    for (int zt : {256, 128, 64, 32, 16}) {
+        if (blocks(zt) >= z_target) {
+            t.byte_zt = zt;
+            break;
+        }
+    }
+    if (t.byte_zt == 0) {
+        t.byte_zt = 256;
+        while (t.byte_zt > 16 && t.byte_zt / 2 >= 4 * kmax) {
+            t.byte_zt /= 2;
+        }
+    }
     // at most kMetroMaxLoci loci per block, so a large batch keeps many blocks
     const long long lpr = std::min<long long>(
         kMetroMaxLoci,
@@ -644,13 +696,13 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         d.rows_per_tile = (int)cdiv(d.G, d.n_it);   // even tiles
         h.z_lds = (size_t)rows * 16 * K * sizeof(uint32_t);
     } else {
-        // one individual pair per lane, 256 pairs per block; short data sets take
-        // 64-locus tiles so that small chains still spread over many blocks; a
-        // chunk of loci's K x A frequencies (FP64) and counts (u32) sit in LDS
-        d.zt_loci = bt.byte_loci > 0 ? std::min(bt.byte_loci, kZLoci) : (L < 16384 ? 64 : kZLoci);
+        // one individual pair per lane, byte_rows (64, 128 or 256) pairs per block,
+        // locus tiles from batch_tiles; a chunk of loci's K x A frequencies (FP64)
+        // and counts (u32) sit in LDS
+        d.zt_loci = bt.byte_loci > 0 ? std::min(bt.byte_loci, kZLoci) : bt.byte_zt;
         d.n_lt = (int)cdiv(L, d.zt_loci);
-        d.rows_per_tile = kByteRows;
-        d.n_it = (int)cdiv(d.G, kByteRows);
+        d.rows_per_tile = bt.byte_rows;
+        d.n_it = (int)cdiv(d.G, bt.byte_rows);
         const size_t per_locus = (size_t)K * A * (sizeof(double) + sizeof(uint32_t));
         size_t chunk = std::max<size_t>(1, std::min<size_t>(kByteLdsMax / per_locus,
                                                             (size_t)d.zt_loci));
@@ -907,6 +959,7 @@ int sgpu_create_batch(const sgpu_model *models, int32_t nchains, const sgpu_data
     BatchTiles bt = batch_tiles(models, data, data_of_chain, nchains, z_target, metro_target);
     bt.byte_loci = tuning ? tuning->byte_loci : 0;
     bt.byte_chunk = tuning ? tuning->byte_chunk : 0;
+    c->byte_rows = bt.byte_rows;
     c->chains.resize(nchains);
     std::vector<double> lam0(nchains);
     for (int ci = 0; ci < nchains; ci++) {
