@@ -65,13 +65,25 @@ struct LikeAcc {
         const double tt = max_raw(t0, c) * max_raw(t1, c);
         prod *= valid ? tt : 1.0;
     }
+    // both copies of one individual, terms given scaled by 2^-32 (exactly: no
+    // subnormals arise); the caller adds 32 to the exponent per multiplied term
+    // (log_value(32 * copies)), so the result equals mul_pair's bit for bit
+    __device__ __forceinline__ void mul_pair_scaled(double g0, double g1, bool valid)
+    {
+        const double c = kSqrtUnder * 0x1p-32;
+        const double tt = max_raw(g0, c) * max_raw(g1, c);
+        prod *= valid ? tt : 1.0;
+    }
     __device__ __forceinline__ void renorm()
     {
         int e;
         prod = frexp(prod, &e);
         expo += e;
     }
-    __device__ __forceinline__ double log_value() const { return log(prod) + (double)expo * kLn2; }
+    __device__ __forceinline__ double log_value(int extra_expo = 0) const
+    {
+        return log(prod) + (double)(expo + extra_expo) * kLn2;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -215,15 +227,18 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                         pre0[k] = s0;
                         pre1[k] = s1;
                     }
-                    if (LIKE) {
-                        acc.mul_pair(x0 ? s1 : s0, x1 ? s1 : s0, valid);
-                    }
                     // r = (w * 2^-32) * S, formed as w * (S * 2^-32): the power-of-two
                     // scaling is exact, so both orders round the same product once.
-                    // Missing copies get r = -1 and never count.
-                    const double h0 = s0 * 0x1p-32, h1 = s1 * 0x1p-32;
-                    const double r0 = valid ? (double)wa * (x0 ? h1 : h0) : -1.0;
-                    const double r1 = valid ? (double)wb * (x1 ? h1 : h0) : -1.0;
+                    // Missing copies scale S by 0 instead: r = 0 never exceeds a prefix
+                    // (all >= 0), so they never count.
+                    const double sc = __hiloint2double(valid ? 0x3DF00000 : 0, 0);   // 2^-32 or 0
+                    const double h0 = s0 * sc, h1 = s1 * sc;
+                    const double g0 = x0 ? h1 : h0, g1 = x1 ? h1 : h0;   // each copy's S * sc
+                    const double r0 = (double)wa * g0;
+                    const double r1 = (double)wb * g1;
+                    if (LIKE) {
+                        acc.mul_pair_scaled(g0, g1, valid);
+                    }
                     // both allele sequences are compared and the copy's allele picks
                     // the lane mask in scalar registers
                     const uint64_t xm0 = ballot(x0), xm1 = ballot(x1);
@@ -298,7 +313,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     }
     if (LIKE) {
         __shared__ double s_red[kZLoci / 64];
-        const double v = lok ? acc.log_value() : 0.0;
+        // every valid copy's term was multiplied in scaled by 2^-32
+        const double v = lok ? acc.log_value(32 * (int)Ttot) : 0.0;
         const double tot = block_sum<kZLoci>(v, s_red);
         if (threadIdx.x == 0) {
             likepart[(size_t)by * n_lt + bx] = tot;
