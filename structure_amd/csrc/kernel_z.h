@@ -66,13 +66,13 @@ struct LikeAcc {
         prod *= valid ? tt : 1.0;
     }
     // both copies of one individual, terms given scaled by 2^-32 (exactly: no
-    // subnormals arise); the caller adds 32 to the exponent per multiplied term
-    // (log_value(32 * copies)), so the result equals mul_pair's bit for bit
-    __device__ __forceinline__ void mul_pair_scaled(double g0, double g1, bool valid)
+    // subnormals arise), a missing individual giving 0, 0 and so the factor
+    // (sqrt(UNDERFLO) 2^-32)^2 without a select; log_value adds 32 to the
+    // exponent per valid copy and takes that factor out once per missing individual
+    __device__ __forceinline__ void mul_pair_scaled(double g0, double g1)
     {
         const double c = kSqrtUnder * 0x1p-32;
-        const double tt = max_raw(g0, c) * max_raw(g1, c);
-        prod *= valid ? tt : 1.0;
+        prod *= max_raw(g0, c) * max_raw(g1, c);
     }
     __device__ __forceinline__ void renorm()
     {
@@ -80,9 +80,10 @@ struct LikeAcc {
         prod = frexp(prod, &e);
         expo += e;
     }
-    __device__ __forceinline__ double log_value(int extra_expo = 0) const
+    __device__ __forceinline__ double log_value(int extra_expo = 0, int missing = 0) const
     {
-        return log(prod) + (double)(expo + extra_expo) * kLn2;
+        const double v = log(prod) + (double)(expo + extra_expo) * kLn2;
+        return missing ? v - (double)(2 * missing) * log(kSqrtUnder * 0x1p-32) : v;
     }
 };
 
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                     const double r0 = (double)wa * g0;
                     const double r1 = (double)wb * g1;
                     if (LIKE) {
-                        acc.mul_pair_scaled(g0, g1, valid);
+                        acc.mul_pair_scaled(g0, g1);
                     }
                     // both allele sequences are compared and the copy's allele picks
                     // the lane mask in scalar registers
@@ -313,8 +314,9 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     }
     if (LIKE) {
         __shared__ double s_red[kZLoci / 64];
-        // every valid copy's term was multiplied in scaled by 2^-32
-        const double v = lok ? acc.log_value(32 * (int)Ttot) : 0.0;
+        // every valid copy's term was multiplied in scaled by 2^-32, every missing
+        // individual as the clamp value squared
+        const double v = lok ? acc.log_value(32 * (int)Ttot, ntile - (int)(Ttot / 2u)) : 0.0;
         const double tot = block_sum<kZLoci>(v, s_red);
         if (threadIdx.x == 0) {
             likepart[(size_t)by * n_lt + bx] = tot;
