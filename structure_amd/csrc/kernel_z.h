@@ -57,14 +57,6 @@ struct LikeAcc {
     double prod = 1.0;
     int expo = 0;
     __device__ __forceinline__ void mul(double t) { prod *= (t > kSqrtUnder) ? t : kSqrtUnder; }
-    // both copies of one individual, branch-free: terms clamped at sqrt(UNDERFLO),
-    // a missing individual contributes 1
-    __device__ __forceinline__ void mul_pair(double t0, double t1, bool valid)
-    {
-        const double c = kSqrtUnder;
-        const double tt = max_raw(t0, c) * max_raw(t1, c);
-        prod *= valid ? tt : 1.0;
-    }
     // both copies of one individual, terms given scaled by 2^-32 (exactly: no
     // subnormals arise), a missing individual giving 0, 0 and so the factor
     // (sqrt(UNDERFLO) 2^-32)^2 without a select; log_value adds 32 to the
