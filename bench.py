@@ -417,7 +417,9 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
                      "traffic": traffic, "kernel": "k_update_z_snp2 (fused UpdateZ + counts + lnP)",
                      "algorithmic_bytes_per_launch": zb, "avg_launch_ms": round(z_avg_s * 1e3, 4),
-                     "launches_timed": zn, "valu": valu},
+                     "launches_timed": zn, "valu": valu,
+                     "traffic_frac": (round(traffic / z_avg_s / 1e9 / PEAK_HBM_GBS, 5)
+                                      if traffic else None)},
         "site_updates_per_s": round(N * L * K * sweeps_per_s, 1),
         "sweep_hbm_frac": round(sweep_bytes(N, L, K, 2) * sweeps_per_s / 1e9 / PEAK_HBM_GBS, 5),
         "phase_ms_per_sweep": {p: round(v[0] / nt, 4) for p, v in timing.items()},

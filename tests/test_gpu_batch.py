@@ -111,3 +111,50 @@ def test_batch_state_is_per_chain():
     out = np.empty(4)
     rc = sa.lib().sgpu_get_chain_state(batch._h, len(CHAINS), sa.Q, out.ctypes.data, out.nbytes)
     assert rc == sa.ERR_ARG
+
+
+def test_ragged_batch_matches_oracle_per_data_set():
+    """Data sets of uneven N, L, format and K in one batch (the config-5 shape,
+    scaled down): for the same (Q, P) state and stream, every chain's UpdateZ
+    counts equal the oracle's exactly and its lnP(D) agrees to 1e-12."""
+    import oracle
+    from helpers import random_p, random_q
+    rng = np.random.default_rng(55)
+    specs = [("snp", 50, 300, 2), ("snp", 130, 1200, 5), ("snp", 77, 517, 8),
+             ("byte", 40, 200, 3), ("byte", 90, 450, 6), ("byte", 61, 333, 4)]
+    dsets, genos = [], []
+    for j, (kind, N, L, _) in enumerate(specs):
+        if kind == "snp":
+            d = sa.synth_snp(N, L, Ktrue=3, qalpha=0.5, miss=0.03, seed=60 + j, want_geno=True)
+            dsets.append((sa.GENO_SNP2, d["words"], d["nall"]))
+            genos.append((d["geno"], d["nall"], 2))
+        else:
+            d = sa.synth_microsat(N, L, amin=2, amax=9, Ktrue=3, miss=0.03, seed=60 + j)
+            fmt, words = sa.pack(d["geno"], sa.GENO_BYTE)
+            dsets.append((fmt, words, d["nall"]))
+            genos.append((d["geno"], d["nall"], d["A"]))
+    models = [sa.make_model(N, L, K, genos[j][2], seed=70 + j, chain=j)
+              for j, (_, N, L, K) in enumerate(specs)]
+    batch = sa.Batch(models, dsets, list(range(len(specs))))
+    batch.init()
+    states = []
+    for j, (_, N, L, K) in enumerate(specs):
+        Q = random_q(rng, N, K, a=0.7)
+        P = random_p(rng, L, K, genos[j][2], genos[j][1], a=1.0)
+        batch.set_chain(j, sa.Q, Q)
+        batch.set_chain(j, sa.P, P)
+        states.append((Q, P))
+    for rep in (5, 6):
+        batch.update_z(rep, like=True)
+        batch.collect(rep)
+        for j, (_, N, L, K) in enumerate(specs):
+            geno, nall, A = genos[j]
+            och = oracle.Chain(oracle.make_model(N, L, K, A, seed=70 + j, chain=j), geno, nall)
+            Q, P = states[j]
+            nA, nQ = och.update_z(Q, P, rep)
+            assert np.array_equal(batch.get_chain(j, sa.NA), nA), (rep, j)
+            assert np.array_equal(batch.get_chain(j, sa.NQ), nQ), (rep, j)
+            like = batch.get_chain(j, sa.LIKE)[0]
+            ref = och.calc_like(Q, P)
+            assert abs(like - ref) <= 1e-12 * abs(ref), (rep, j, like, ref)
+    batch.close()
