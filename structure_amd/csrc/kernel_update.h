@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ 
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const SGPU_G int32_t *nall = g(C.nall);
-    const SGPU_G int32_t *nApart = g(C.nApart);
+    const SGPU_G uint16_t *nApart = g(C.nApart);
     const SGPU_G double *eps = g(C.eps);
     const SGPU_G double *F = g(C.F);
     const SGPU_G double *lambda = g(C.lambda);
@@ -90,14 +90,14 @@ __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ 
         int n0 = 0, n1 = 0, n2 = 0, n3 = 0;
         int it = 0;
         for (; it + 8 <= n_it; it += 8) {
-            const SGPU_G int32_t *q = nApart + it * stride + pos;
+            const SGPU_G uint16_t *q = nApart + it * stride + pos;
             n0 += q[0] + q[4 * stride];
             n1 += q[stride] + q[5 * stride];
             n2 += q[2 * stride] + q[6 * stride];
             n3 += q[3 * stride] + q[7 * stride];
         }
         if (it + 4 <= n_it) {
-            const SGPU_G int32_t *q = nApart + it * stride + pos;
+            const SGPU_G uint16_t *q = nApart + it * stride + pos;
             n0 += q[0];
             n1 += q[stride];
             n2 += q[2 * stride];
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kNT) void k_fold_rows(const ChainDesc *__restrict__
 {
     SGPU_BLOCK_PROLOGUE();
     const int by = (int)br_.by;
-    const SGPU_G int32_t *part = g(C.nQpart);
+    const SGPU_G uint16_t *part = g(C.nQpart);
     SGPU_G int32_t *out = g(C.nQfold);
     const int nrows = C.n_lt;
     const size_t n = (size_t)m.N * m.K;
@@ -1022,7 +1022,7 @@ __global__ __launch_bounds__(kNT) void k_sweep_tail(const ChainDesc *__restrict_
 }
 
 // Sum the per-tile count partials into full arrays (for state reads).
-__global__ void k_reduce_counts(const int32_t *__restrict__ part, int nparts, size_t n,
+__global__ void k_reduce_counts(const uint16_t *__restrict__ part, int nparts, size_t n,
                                 int32_t *__restrict__ out)
 {
     const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

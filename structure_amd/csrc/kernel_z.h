@@ -112,8 +112,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     const __attribute__((address_space(4))) double *Q =
         (const __attribute__((address_space(4))) double *)C.Q;
     const SGPU_G double *P = g(C.P);
-    SGPU_G int32_t *nApart = g(C.nApart);
-    SGPU_G int32_t *nQpart = g(C.nQpart);
+    SGPU_G uint16_t *nApart = g(C.nApart);
+    SGPU_G uint16_t *nQpart = g(C.nQpart);
     SGPU_G double *likepart = g(C.likepart);
     SGPU_G int8_t *zout = g(C.zout);
     extern __shared__ uint32_t s_cnt[];   // [tile individual][K]: valid copies, then z > k
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     }
 
     if (lok) {
-        int32_t *dst = nApart + ((size_t)by * L + l) * K * A;
+        SGPU_G uint16_t *dst = nApart + ((size_t)by * L + l) * K * A;
         uint32_t prevT = Ttot, prevA = A1tot;
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -287,8 +287,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
             const uint32_t ak = (k < K - 1) ? A1[k] : 0u;
             const uint32_t n1 = prevA - ak;
             const uint32_t nt = prevT - tk;
-            dst[k * A + 0] = (int32_t)(nt - n1);
-            dst[k * A + 1] = (int32_t)n1;
+            dst[k * A + 0] = (uint16_t)(nt - n1);
+            dst[k * A + 1] = (uint16_t)n1;
             prevT = tk;
             prevA = ak;
         }
@@ -296,11 +296,11 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     __syncthreads();
     // convert "valid, z > 0, ..., z > K-2" into counts per population
     for (int t = threadIdx.x; t < ntile; t += kZLoci) {
-        int32_t *dst = nQpart + ((size_t)bx * N + ibase + t) * K;
+        SGPU_G uint16_t *dst = nQpart + ((size_t)bx * N + ibase + t) * K;
         uint32_t prev = s_cnt[t * K + 0];
         for (int k = 0; k < K; k++) {
             const uint32_t gk = (k < K - 1) ? s_cnt[t * K + 1 + k] : 0u;
-            dst[k] = (int32_t)(prev - gk);
+            dst[k] = (uint16_t)(prev - gk);
             prev = gk;
         }
     }
@@ -386,8 +386,8 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
     const SGPU_G uint32_t *geno = g(C.geno);
     const SGPU_G double *Q = g(C.Q);
     const SGPU_G double *P = g(C.P);
-    SGPU_G int32_t *nApart = g(C.nApart);
-    SGPU_G int32_t *nQpart = g(C.nQpart);
+    SGPU_G uint16_t *nApart = g(C.nApart);
+    SGPU_G uint16_t *nQpart = g(C.nQpart);
     SGPU_G double *likepart = g(C.likepart);
     SGPU_G int8_t *zout = g(C.zout);
     extern __shared__ double s_dyn[];
@@ -470,29 +470,29 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
         }
         __syncthreads();
         // this block alone covers (individual tile by, loci c0..c0+nc): plain stores
-        SGPU_G int32_t *dst = nApart + ((size_t)by * L + c0) * KA;
+        SGPU_G uint16_t *dst = nApart + ((size_t)by * L + c0) * KA;
         for (int t = threadIdx.x; t < nc * KA; t += R) {
-            dst[t] = (int32_t)s_h[t];
+            dst[t] = (uint16_t)s_h[t];
         }
     }
     // counts per population from the "z > k" counts
     if (ok0) {
-        SGPU_G int32_t *d0 = nQpart + ((size_t)bx * N + i0) * K;
+        SGPU_G uint16_t *d0 = nQpart + ((size_t)bx * N + i0) * K;
         uint32_t prev = V0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const uint32_t tk = (k < K - 1) ? T0[k] : 0u;
-            d0[k] = (int32_t)(prev - tk);
+            d0[k] = (uint16_t)(prev - tk);
             prev = tk;
         }
     }
     if (ok1) {
-        SGPU_G int32_t *d1 = nQpart + ((size_t)bx * N + i1) * K;
+        SGPU_G uint16_t *d1 = nQpart + ((size_t)bx * N + i1) * K;
         uint32_t prev = V1;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const uint32_t tk = (k < K - 1) ? T1[k] : 0u;
-            d1[k] = (int32_t)(prev - tk);
+            d1[k] = (uint16_t)(prev - tk);
             prev = tk;
         }
     }

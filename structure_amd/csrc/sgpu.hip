@@ -689,7 +689,9 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         d.z_chunk = 0;
         d.n_lt = (int)cdiv(L, kZLoci);
         int rows = std::min(bt.snp_rows, d.G);
-        while ((size_t)rows * 16 * K * 4 > kZLdsMax && rows > 1) {
+        // LDS counts <= kZLdsMax; that also keeps a tile's copies (rows x 32) within
+        // the 16-bit count partials
+        while (((size_t)rows * 16 * K * 4 > kZLdsMax || rows * 32 > 65535) && rows > 1) {
             rows = (rows + 1) / 2;
         }
         d.n_it = (int)cdiv(d.G, rows);
@@ -1134,11 +1136,26 @@ int sgpu_set_chain_state(sgpu_ctx *c, int32_t chain, int32_t which, const void *
         return SGPU_OK;
     }
     if (which == SGPU_NA || which == SGPU_NQ) {
+        // the 16-bit per-tile partials take the totals, up to 65535 per partial
         const bool isA = which == SGPU_NA;
-        int32_t *part = isA ? h.d.nApart : h.d.nQpart;
+        uint16_t *part = isA ? h.d.nApart : h.d.nQpart;
         const int nparts = isA ? h.d.n_it : h.d.n_lt;
-        HIP_OK(hipMemsetAsync(part, 0, (size_t)nparts * n * sizeof(int32_t), c->stream));
-        HIP_OK(hipMemcpyAsync(part, host, bytes, hipMemcpyHostToDevice, c->stream));
+        const int32_t *tot = (const int32_t *)host;
+        std::vector<uint16_t> hp((size_t)nparts * n, 0);
+        for (size_t j = 0; j < n; j++) {
+            if (tot[j] < 0 || (int64_t)tot[j] > (int64_t)65535 * nparts) {
+                return fail(SGPU_ERR_ARG, "count %d at %zu outside [0, %lld]", tot[j], j,
+                            (long long)65535 * nparts);
+            }
+            int32_t left = tot[j];
+            for (int t = 0; left > 0; t++) {
+                const int32_t v = std::min(left, 65535);
+                hp[(size_t)t * n + j] = (uint16_t)v;
+                left -= v;
+            }
+        }
+        HIP_OK(hipMemcpyAsync(part, hp.data(), hp.size() * sizeof(uint16_t),
+                              hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
         return SGPU_OK;
     }

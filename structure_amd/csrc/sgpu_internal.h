@@ -54,8 +54,10 @@ struct ChainDesc {
     const int32_t *nall;           // [L]
     const int32_t *ahist;          // (allele count, loci) pairs
     double *Q, *P, *eps, *F, *alpha, *lambda;
-    int32_t *nApart;               // [n_it][L][K][A]
-    int32_t *nQpart;               // [n_lt][N][K]
+    // count partials of one UpdateZ tile each: at most 24576 copies per tile
+    // (the LDS cap on a 2-bit tile's rows), so 16 bits hold them
+    uint16_t *nApart;              // [n_it][L][K][A]
+    uint16_t *nQpart;              // [n_lt][N][K]
     int32_t *nQfold;               // [16][N][K]
     int32_t *logq_rows;            // rows of logq_part written by the last Q update
     double *likepart;              // [n_it][n_lt]
