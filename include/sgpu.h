@@ -226,9 +226,10 @@ int sgpu_synchronize(sgpu_ctx *ctx);
 /* Diagnostics, no counterpart in the reference: y[i] = f(x[i]) evaluated by the
  * device code the sweep uses (host arrays in and out; a temporary device
  * allocation on the current device).  SGPU_FN_LOG: natural logarithm;
- * SGPU_FN_COS: cosine for 0 <= x <= 8 (the Box-Muller angle); SGPU_FN_LGAMMA:
- * ln Gamma(x), x > 0, by the Lanczos sum of mylgamma (mymath.c:382-429). */
-enum { SGPU_FN_LOG = 0, SGPU_FN_COS = 1, SGPU_FN_LGAMMA = 2 };
+ * SGPU_FN_COS / SGPU_FN_SIN: cosine / sine for 0 <= x <= 8 (the Box-Muller
+ * angle); SGPU_FN_LGAMMA: ln Gamma(x), x > 0, by the Lanczos sum of mylgamma
+ * (mymath.c:382-429). */
+enum { SGPU_FN_LOG = 0, SGPU_FN_COS = 1, SGPU_FN_LGAMMA = 2, SGPU_FN_SIN = 3 };
 int sgpu_eval_math(int32_t fn, const double *x, double *y, int64_t n);
 
 /* Message for the last failing call on this thread. */

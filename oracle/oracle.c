@@ -20,7 +20,8 @@
 #define ORC_SQRTUNDER 1e-50           /* sqrt(UNDERFLO), CalcLike.c:99 */
 #define ORC_LAMBDAMAX 10.0            /* structure.h:17 */
 #define ORC_LAMBDAPROPSD 0.3          /* structure.h:18 */
-#define ORC_GAMMA_MAX_ATTEMPTS 256u
+#define ORC_GAMMA_MAX_BLOCKS 128u          /* 256 attempts */
+#define ORC_GAMMA_BOOST_BLOCK 0xFFFFFFu
 
 /* ------------------------------------------------------------------------
  * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
@@ -74,9 +75,29 @@ double orc_normal(uint32_t seed, uint32_t chain, uint32_t c0, uint32_t c1, uint3
     return orc_box_muller(w[0], w[1]);
 }
 
+/* One Marsaglia-Tsang attempt with normal x and acceptance uniform u. */
+static int orc_mt_attempt(double x, double u, double d, double c, double *g)
+{
+    double v = 1.0 + c * x;
+    if (v <= 0.0) {
+        return 0;
+    }
+    v = v * v * v;
+    double x2 = x * x;
+    if (u < 1.0 - 0.0331 * (x2 * x2) || log(u) < 0.5 * x2 + d * (1.0 - v + log(v))) {
+        *g = d * v;
+        return 1;
+    }
+    return 0;
+}
+
 /* Gamma(a, 1) by Marsaglia & Tsang (2000); shape < 1 via Gamma(a+1) * U^(1/a).
  * Replaces the reference samplers RGamma (ran.c:168-261) and RGammaDisc
- * (ran.c:1094-1188), which draw the same distribution by Ripley/Cheng GKM. */
+ * (ran.c:1094-1188), which draw the same distribution by Ripley/Cheng GKM.
+ * Philox block j serves two attempts: words 0,1 give a Box-Muller pair, the
+ * cos normal is tried with the uniform of word 2, then the sin normal with the
+ * uniform of word 3; the boost uniform for a < 1 is word 0 of block
+ * ORC_GAMMA_BOOST_BLOCK. */
 double orc_gamma(double a, uint32_t seed, uint32_t chain, uint32_t c0, uint32_t c1,
                  uint32_t c2, uint32_t phase)
 {
@@ -86,26 +107,25 @@ double orc_gamma(double a, uint32_t seed, uint32_t chain, uint32_t c0, uint32_t 
     double aa = a < 1.0 ? a + 1.0 : a;
     double d = aa - 1.0 / 3.0;
     double c = 1.0 / sqrt(9.0 * d);
-    for (uint32_t j = 0; j < ORC_GAMMA_MAX_ATTEMPTS; j++) {
+    double g = 0.0;
+    int done = 0;
+    for (uint32_t j = 0; j < ORC_GAMMA_MAX_BLOCKS && !done; j++) {
         uint32_t w[4];
         orc_philox_block(seed, chain, c0, c1, c2, (phase << 24) | j, w);
-        double x = orc_box_muller(w[0], w[1]);
-        double v = 1.0 + c * x;
-        if (v <= 0.0) {
-            continue;
-        }
-        v = v * v * v;
-        double u = orc_uo(w[2]);
-        double x2 = x * x;
-        if (u < 1.0 - 0.0331 * (x2 * x2) || log(u) < 0.5 * x2 + d * (1.0 - v + log(v))) {
-            double g = d * v;
-            if (a < 1.0) {
-                g = g * exp(log(orc_uo(w[3])) / a);
-            }
-            return g;
-        }
+        double rad = sqrt(-2.0 * log(orc_uo(w[0])));
+        double th = 6.283185307179586 * orc_uo(w[1]);
+        done = orc_mt_attempt(rad * cos(th), orc_uo(w[2]), d, c, &g) ||
+               orc_mt_attempt(rad * sin(th), orc_uo(w[3]), d, c, &g);
     }
-    return d;   /* unreachable in practice: acceptance >= 0.95 per attempt */
+    if (!done) {
+        return d;   /* 256 rejections: unreachable in practice (acceptance >= 0.95) */
+    }
+    if (a < 1.0) {
+        uint32_t b[4];
+        orc_philox_block(seed, chain, c0, c1, c2, (phase << 24) | ORC_GAMMA_BOOST_BLOCK, b);
+        g = g * exp(log(orc_uo(b[0])) / a);
+    }
+    return g;
 }
 
 /* ln Gamma(z), z > 0: Lanczos (g = 7, n = 9) with the published coefficient

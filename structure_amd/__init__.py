@@ -132,11 +132,11 @@ def _vp(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
-FN_LOG, FN_COS, FN_LGAMMA = 0, 1, 2
+FN_LOG, FN_COS, FN_LGAMMA, FN_SIN = 0, 1, 2, 3
 
 
 def eval_math(fn: int, x) -> np.ndarray:
-    """The device's own log / cos / lgamma over an array (diagnostics)."""
+    """The device's own log / cos / sin / lgamma over an array (diagnostics)."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.empty_like(x)
     _check(lib().sgpu_eval_math(fn, _vp(x), _vp(y), x.size))

@@ -116,13 +116,14 @@ __device__ __forceinline__ double log_d(double x)
     return x == 0.0 ? -__builtin_inf() : (x == __builtin_inf() ? x : __builtin_nan(""));
 }
 
-// cos a for 0 <= a <= 8 (the Box-Muller angle).  a = q pi/2 + r with
-// q = rint(2a/pi) and r from a two-part pi/2 (Cody-Waite; the head product is
-// exact under the fused multiply-add, so r keeps its relative accuracy next
-// to the zeros of cos); cos r and sin r by their Taylor series through r^16
-// and r^17 (|r| <= pi/4: truncation below 2^-60), quadrant by q mod 4.
+// sin a and cos a for 0 <= a <= 8 (the Box-Muller angle).  a = q pi/2 + r
+// with q = rint(2a/pi) and r from a two-part pi/2 (Cody-Waite; the head
+// product is exact under the fused multiply-add, so r keeps its relative
+// accuracy next to the zeros); cos r and sin r by their Taylor series through
+// r^16 and r^17 (|r| <= pi/4: truncation below 2^-60), quadrant by q mod 4:
+// cos a = (c, -s, -c, s)[q mod 4], sin a = (s, c, -s, -c)[q mod 4].
 // Error about 1 ulp.
-__device__ __forceinline__ double cos_d(double a)
+__device__ __forceinline__ void sincos_d(double a, double *sn, double *cs)
 {
     const double q = rint(a * 0.6366197723675814);
     double r = fma(-q, 1.5707963267948966, a);
@@ -147,8 +148,17 @@ __device__ __forceinline__ double cos_d(double a)
     t = fma(t, z, 0.008333333333333333);
     t = fma(t, z, -0.16666666666666666);
     t = fma(t * z, r, r);
-    const double v = (n & 1) ? t : c;
-    return (n == 1 || n == 2) ? -v : v;
+    const double cv = (n & 1) ? t : c;
+    const double sv = (n & 1) ? c : t;
+    *cs = (n == 1 || n == 2) ? -cv : cv;
+    *sn = (n >= 2) ? -sv : sv;
+}
+
+__device__ __forceinline__ double cos_d(double a)
+{
+    double sn, cs;
+    sincos_d(a, &sn, &cs);
+    return cs;
 }
 
 __device__ __forceinline__ double box_muller(uint32_t w0, uint32_t w1)
@@ -164,12 +174,31 @@ __device__ __forceinline__ double normal_draw(uint32_t seed, uint32_t chain, uin
     return box_muller(w.x, w.y);
 }
 
-constexpr uint32_t kGammaMaxAttempts = 256;
+constexpr uint32_t kGammaMaxBlocks = 128;        // 256 attempts
+constexpr uint32_t kGammaBoostBlock = 0xFFFFFFu;   // attempt field of the shape < 1 boost
+
+// One Marsaglia-Tsang attempt with normal x and acceptance uniform u.
+__device__ __forceinline__ bool mt_attempt(double x, double u, double d, double c, double *g)
+{
+    double v = 1.0 + c * x;
+    if (v <= 0.0) {
+        return false;
+    }
+    v = v * v * v;
+    const double x2 = x * x;
+    if (u < 1.0 - 0.0331 * (x2 * x2) || log_d(u) < 0.5 * x2 + d * (1.0 - v + log_d(v))) {
+        *g = d * v;
+        return true;
+    }
+    return false;
+}
 
 // Gamma(a, 1), Marsaglia & Tsang (ACM TOMS 26(3), 2000); a < 1 through
-// Gamma(a+1) * U^(1/a).  One Philox block per attempt: words 0,1 -> normal,
-// 2 -> acceptance uniform, 3 -> boost uniform.  FP64 throughout: with
-// shapes in the thousands (allele / ancestry counts) the acceptance test
+// Gamma(a+1) * U^(1/a).  Philox block j (attempt field j) serves two
+// attempts: words 0,1 give a Box-Muller pair, the cos normal tried with the
+// uniform of word 2, then the sin normal with the uniform of word 3; the boost
+// uniform for a < 1 is word 0 of block kGammaBoostBlock.  FP64 throughout:
+// with shapes in the thousands (allele / ancestry counts) the acceptance test
 // d*(1 - v + ln v) cancels catastrophically in FP32.
 __device__ __forceinline__ double gamma_draw(double a, uint32_t seed, uint32_t chain,
                                              uint32_t c0, uint32_t c1, uint32_t c2,
@@ -181,25 +210,23 @@ __device__ __forceinline__ double gamma_draw(double a, uint32_t seed, uint32_t c
     const double aa = a < 1.0 ? a + 1.0 : a;
     const double d = aa - 1.0 / 3.0;
     const double c = 1.0 / sqrt(9.0 * d);
-    for (uint32_t j = 0; j < kGammaMaxAttempts; j++) {
-        u32x4 w = philox4x32_10(c0, c1, c2, (phase << 24) | j, seed, chain);
-        const double x = box_muller(w.x, w.y);
-        double v = 1.0 + c * x;
-        if (v <= 0.0) {
-            continue;
-        }
-        v = v * v * v;
-        const double u = uo(w.z);
-        const double x2 = x * x;
-        if (u < 1.0 - 0.0331 * (x2 * x2) || log_d(u) < 0.5 * x2 + d * (1.0 - v + log_d(v))) {
-            double g = d * v;
-            if (a < 1.0) {
-                g = g * exp(log_d(uo(w.w)) / a);
-            }
-            return g;
-        }
+    double g = 0.0;
+    bool done = false;
+    for (uint32_t j = 0; j < kGammaMaxBlocks && !done; j++) {
+        const u32x4 w = philox4x32_10(c0, c1, c2, (phase << 24) | j, seed, chain);
+        const double rad = sqrt(-2.0 * log_d(uo(w.x)));
+        double sn, cs;
+        sincos_d(6.283185307179586 * uo(w.y), &sn, &cs);
+        done = mt_attempt(rad * cs, uo(w.z), d, c, &g) || mt_attempt(rad * sn, uo(w.w), d, c, &g);
     }
-    return d;
+    if (!done) {
+        return d;   // 256 rejections: unreachable in practice (acceptance >= 0.95)
+    }
+    if (a < 1.0) {
+        const u32x4 b = philox4x32_10(c0, c1, c2, (phase << 24) | kGammaBoostBlock, seed, chain);
+        g = g * exp(log_d(uo(b.x)) / a);
+    }
+    return g;
 }
 
 // Term-by-term Lanczos sum in the host routine's order (mymath.c:382-429).

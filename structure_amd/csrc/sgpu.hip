@@ -431,7 +431,10 @@ __global__ void k_eval_math(int fn, const double *__restrict__ x, double *__rest
         return;
     }
     const double v = x[i];
-    y[i] = fn == SGPU_FN_LOG ? log_d(v) : (fn == SGPU_FN_COS ? cos_d(v) : lgamma_lanczos(v));
+    double sn, cs;
+    sincos_d(v, &sn, &cs);
+    y[i] = fn == SGPU_FN_LOG ? log_d(v)
+                             : (fn == SGPU_FN_COS ? cs : (fn == SGPU_FN_SIN ? sn : lgamma_lanczos(v)));
 }
 
 struct DevBuf {
@@ -1455,7 +1458,7 @@ int sgpu_get_timing(sgpu_ctx *c, double *ms, int64_t *launches)
 
 int sgpu_eval_math(int32_t fn, const double *x, double *y, int64_t n)
 {
-    if (fn < SGPU_FN_LOG || fn > SGPU_FN_LGAMMA || n < 0 || (n > 0 && (!x || !y))) {
+    if (fn < SGPU_FN_LOG || fn > SGPU_FN_SIN || n < 0 || (n > 0 && (!x || !y))) {
         return fail(SGPU_ERR_ARG, "sgpu_eval_math: fn %d, n %lld", fn, (long long)n);
     }
     if (n == 0) {

@@ -1,8 +1,8 @@
 """The device special functions of the sampler against the host's libm.
 
-log_d and cos_d are the sampler's own (atanh series / Taylor series with
-Cody-Waite reduction); they must stay within 2 ulp of glibc, whose log and cos
-are within an ulp of the true values.  lgamma_lanczos is the reference's
+log_d and sincos_d are the sampler's own (atanh series / Taylor series with
+Cody-Waite reduction); they must stay within 2 ulp of glibc, whose log, cos
+and sin are within an ulp of the true values.  lgamma_lanczos is the reference's
 Lanczos sum (mymath.c:382-429): against the oracle's term-by-term restatement
 within a few ulps, and against math.lgamma within the approximation's own
 error
@@ -44,7 +44,7 @@ def test_log_within_two_ulps():
     assert np.isnan(sp[2]) and np.isnan(sp[3])
 
 
-def test_cos_of_box_muller_angles():
+def test_cos_and_sin_of_box_muller_angles():
     rng = np.random.default_rng(2)
     w = np.concatenate([
         rng.integers(0, 2**32, 300_000, dtype=np.uint64),
@@ -55,6 +55,9 @@ def test_cos_of_box_muller_angles():
     a = 6.283185307179586 * ((w.astype(np.float64) + 0.5) * 2.0**-32)
     y = sa.eval_math(sa.FN_COS, a)
     ref = np.array([math.cos(v) for v in a])
+    assert _ulps(y, ref).max() <= 2.0
+    y = sa.eval_math(sa.FN_SIN, a)
+    ref = np.array([math.sin(v) for v in a])
     assert _ulps(y, ref).max() <= 2.0
 
 
