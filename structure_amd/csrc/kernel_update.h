@@ -50,6 +50,33 @@ __device__ __forceinline__ double wave_strided_sum(const SGPU_G double *x, int n
     return wave_sum(v);
 }
 
+// Column sums of an [n][ncol] row-major table (ncol <= kMaxK + 1) by the 64
+// lanes of one wave: lane j takes rows j, j + 64, ... with every column's load
+// of a row issued together, then one shuffle tree per column.  out[c] is valid
+// in every lane (broadcast from lane 0).
+__device__ __forceinline__ void wave_col_sums(const SGPU_G double *x, int n, int ncol,
+                                              double *out /* [kMaxK + 1] */)
+{
+    double a[kMaxK + 1];
+#pragma unroll
+    for (int c = 0; c <= kMaxK; c++) {
+        a[c] = 0.0;
+    }
+    for (int b = (int)(threadIdx.x & 63); b < n; b += 64) {
+        const SGPU_G double *row = x + (size_t)b * ncol;
+#pragma unroll
+        for (int c = 0; c <= kMaxK; c++) {
+            if (c < ncol) {
+                a[c] += row[c];
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c <= kMaxK; c++) {
+        out[c] = c < ncol ? __shfl(wave_sum(a[c]), 0, 64) : 0.0;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // UpdateP (UpdateP.c:106-199): one thread per (locus, pop, allele), a block
 // holding whole (locus, pop) groups (A * (256 / A) threads).  Dirichlet
@@ -565,14 +592,16 @@ __device__ void alpha_wave(const ChainDesc &C, const ModelArgs &m, uint32_t rep)
     SGPU_G double *alpha = g(C.alpha);
     const int nparts = g(C.logq_rows)[0];
     __shared__ double s_stat[kMaxK + 1];   // written and read by lane 0 only
-    for (int k = 0; k <= m.K; k++) {
-        const double t = wave_strided_sum(logq_part, nparts, m.K + 1, k);
-        if ((threadIdx.x & 63) == 0) {
-            s_stat[k] = t;
+    {
+        double sums[kMaxK + 1];
+        wave_col_sums(logq_part, nparts, m.K + 1, sums);
+        if ((threadIdx.x & 63) != 0) {
+            return;
         }
-    }
-    if ((threadIdx.x & 63) != 0) {
-        return;
+#pragma unroll
+        for (int c = 0; c <= kMaxK; c++) {
+            s_stat[c] = sums[c];
+        }
     }
     const int K = m.K;
     const double N = (double)m.N;
@@ -897,10 +926,12 @@ __device__ void fst_final_wave(const ChainDesc &C, const ModelArgs &m, uint32_t 
     // the proposal groups are independent (each reads and writes its own F), so
     // lane p1 decides group p1 with the group's sum of block partials
     const int p1 = (int)(threadIdx.x & 63);
+    double sums[kMaxK + 1];
+    wave_col_sums(part, nparts, numpops1, sums);
     double stat = 0.0;
-    for (int q = 0; q < numpops1; q++) {
-        const double t = __shfl(wave_strided_sum(part, nparts, numpops1, q), 0, 64);
-        stat = p1 == q ? t : stat;
+#pragma unroll
+    for (int q = 0; q < kMaxK; q++) {
+        stat = p1 == q ? sums[q] : stat;
     }
     if (p1 >= numpops1) {
         return;
@@ -941,23 +972,30 @@ __global__ __launch_bounds__(kNT) void k_fst_finalize(const ChainDesc *__restric
 // and lnP(D) from the UpdateZ partials (one block per chain, fixed order).
 // ---------------------------------------------------------------------------
 // lnP(D) of the last UpdateZ: the sum of its tile partials by one wave (lane 0).
-__device__ __forceinline__ double like_wave(const ChainDesc &C)
+// lnP(D) of the last UpdateZ is the sum of its tile partials, split in two
+// halves summed by two waves (lane 0 of each), the halves then added in order.
+__device__ __forceinline__ double like_half(const ChainDesc &C, int half)
 {
-    return wave_strided_sum(g(C.likepart), C.n_it * C.n_lt, 1, 0);
+    const int n = C.n_it * C.n_lt, h = (n + 1) / 2;
+    return half ? wave_strided_sum(g(C.likepart) + h, n - h, 1, 0)
+                : wave_strided_sum(g(C.likepart), h, 1, 0);
 }
 
-// The scalar posterior sums and the lnP accumulators, one thread.
+// The scalar posterior sums and the lnP accumulators, by the lanes of one
+// wave: lane k adds pop k's lambda, alpha and F (independent loads, no chain
+// of dependent read-modify-writes), lane 0 the lnP accumulators.
 __device__ __forceinline__ void collect_scalars(const ChainDesc &C, const ModelArgs &m,
                                                 bool with_like, double like)
 {
-    for (int k = 0; k < m.K; k++) {
+    const int k = (int)(threadIdx.x & 63);
+    if (k < m.K) {
         g(C.lambdaSum)[k] += g(C.lambda)[k];
         g(C.alphaSum)[k] += g(C.alpha)[k];
         if (m.freqscorr) {
             g(C.FstSum)[k] += g(C.F)[k];
         }
     }
-    if (with_like) {
+    if (with_like && k == 0) {
         SGPU_G double *like3 = g(C.like3);
         like3[0] = like;
         like3[1] += like;
@@ -965,13 +1003,21 @@ __device__ __forceinline__ void collect_scalars(const ChainDesc &C, const ModelA
     }
 }
 
-__global__ __launch_bounds__(64) void k_collect(const ChainDesc *__restrict__ cds,
-                                                const BlockRef *__restrict__ map, int with_like)
+__global__ __launch_bounds__(128) void k_collect(const ChainDesc *__restrict__ cds,
+                                                 const BlockRef *__restrict__ map, int with_like)
 {
     SGPU_BLOCK_PROLOGUE();
-    const double like = with_like ? like_wave(C) : 0.0;
-    if (threadIdx.x == 0) {
-        collect_scalars(C, m, with_like != 0, like);
+    __shared__ double s_half[2];
+    const int wid = (int)(threadIdx.x >> 6);
+    if (with_like) {
+        const double v = like_half(C, wid);
+        if ((threadIdx.x & 63) == 0) {
+            s_half[wid] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        collect_scalars(C, m, with_like != 0, with_like ? s_half[0] + s_half[1] : 0.0);
     }
 }
 
@@ -990,23 +1036,23 @@ __global__ __launch_bounds__(kNT) void k_sweep_tail(const ChainDesc *__restrict_
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repd;
     const int wid = (int)(threadIdx.x >> 6);
-    __shared__ double s_like;
-    // three independent waves: alpha, the Fst decision, the lnP sum
+    __shared__ double s_half[2];
+    // independent waves: alpha, the Fst decision, and the two halves of the lnP sum
     if (wid == 0 && (flags & kTailAlpha) && m.inferalpha) {
         alpha_wave(C, m, rep);
     }
     if (wid == 1 && (flags & kTailFst) && m.freqscorr) {
         fst_final_wave(C, m, rep);
     }
-    if (wid == 2 && (flags & kTailLike)) {
-        const double like = like_wave(C);
+    if (wid >= 2 && (flags & kTailLike)) {
+        const double v = like_half(C, wid - 2);
         if ((threadIdx.x & 63) == 0) {
-            s_like = like;
+            s_half[wid - 2] = v;
         }
     }
-    __syncthreads();   // alpha, F and the lnP sum are read by thread 0 below
-    if ((flags & kTailCollect) && threadIdx.x == 0) {
-        collect_scalars(C, m, (flags & kTailLike) != 0, s_like);
+    __syncthreads();   // alpha, F and the lnP sum are read by wave 0 below
+    if ((flags & kTailCollect) && threadIdx.x < 64) {
+        collect_scalars(C, m, (flags & kTailLike) != 0, s_half[0] + s_half[1]);
     }
     if (flags & kTailAdvance) {
         __syncthreads();

@@ -1314,7 +1314,7 @@ int sgpu_collect(sgpu_ctx *c, uint32_t rep)
     }
     PhaseTimer timer(c, SGPU_PH_COLLECT);
     const int with_like = (c->computeprob && c->like_ready) ? 1 : 0;
-    if ((rc = launch(c, k_collect, c->p_collect, 64, with_like))) {
+    if ((rc = launch(c, k_collect, c->p_collect, 128, with_like))) {
         return rc;
     }
     c->nsum++;
