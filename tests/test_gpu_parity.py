@@ -274,6 +274,45 @@ def test_chain_trajectory_matches_oracle(kind):
     assert gch.get(sa.NSUM)[0] == och.state.nsum == 23
 
 
+TINY = [  # (format, N, L, K): single individuals and loci, odd N, L off every tile size
+    ("snp", 1, 1, 1), ("snp", 1, 5, 2), ("snp", 3, 1, 3), ("snp", 17, 3, 2),
+    ("snp", 33, 257, 4), ("snp", 2, 300, 16),
+    ("byte", 1, 1, 2), ("byte", 3, 7, 3), ("byte", 5, 129, 5), ("byte", 2, 33, 16),
+]
+
+
+@pytest.mark.parametrize("fmt,N,L,K", TINY)
+def test_tiny_and_odd_shapes_follow_the_oracle(fmt, N, L, K):
+    """Edge shapes through whole sweeps (Gibbs and Metropolis Q, F model, alpha
+    and lambda) with graph replay: the same trajectory as the oracle."""
+    rng = np.random.default_rng(N * 1000 + L * 10 + K)
+    if fmt == "snp":   # 2-bit words: biallelic, both copies missing together
+        geno, nall = random_geno(rng, N, L, 2, miss=0.0, biallelic=True)
+        geno = np.sort(geno, axis=1)   # heterozygotes as (0, 1)
+        gone = rng.random((N, L)) < 0.1
+        for c in range(2):
+            geno[:, c, :] = np.where(gone, -9, geno[:, c, :])
+        A, f = 2, sa.GENO_SNP2
+    else:
+        geno, nall = random_geno(rng, N, L, 5, miss=0.1)
+        A, f = 5, sa.GENO_BYTE
+    kw = dict(freqscorr=1, inferalpha=1, inferlambda=1, burnin=3, metrofreq=2)
+    och, gch = _pair(geno, nall, K, A, fmt=f, seed=99, chain=K, **kw)
+    assert gch.fmt == f
+    gch.init()
+    och.init()
+    gch.run(0, 7)
+    for rep in range(7):
+        och.sweep(rep)
+    assert np.array_equal(gch.get(sa.NA), och.nA)
+    assert np.array_equal(gch.get(sa.NQ), och.nQ)
+    for sel, ref in [(sa.Q, och.Q), (sa.P, och.P), (sa.EPS, och.eps), (sa.FST, och.F),
+                     (sa.ALPHA, och.alpha), (sa.LAMBDA, och.lambda_), (sa.QSUM, och.QSum),
+                     (sa.PSUM, och.PSum)]:
+        _close(gch.get(sel), ref, 1e-9)
+    _close(gch.get(sa.LIKE), [och.like, och.sumlikes, och.sumsqlikes], 1e-9)
+
+
 @pytest.mark.parametrize("variant", ["full", "no_tail_in_burnin"])
 def test_graph_replays_match_direct_launches(variant):
     """sgpu_run_sweeps replays captured HIP graphs (one per sweep type) with the
