@@ -26,12 +26,11 @@ namespace sgpu {
 // Lane mask of a predicate (the condition's own mask, no VGPR round trip).
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
-// v + (bit of this lane in mask): one carry-in add, the mask staying in SGPRs.
-__device__ __forceinline__ uint32_t add_lane_bit(uint32_t v, uint64_t mask)
+// (bit of this lane in mask) ? v : 0, the mask staying in SGPRs.
+__device__ __forceinline__ uint32_t lane_select(uint64_t mask, uint32_t v)
 {
     uint32_t r;
-    uint64_t carry_out;
-    asm("v_addc_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(r), "=s"(carry_out) : "v"(v), "s"(mask));
+    asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(r) : "v"(v), "s"(mask));
     return r;
 }
 
@@ -85,8 +84,12 @@ struct LikeAcc {
 // Per lane (locus) and individual, with the pop-k prefix sums pre_a[k] of
 // allele a and r_c = u_c * S_c for copy c:  z_c > k  <=>  pre_{x_c}[k] < r_c.
 // The kernel only ever forms these "z > k" predicates:
-//   per-locus counters   T[k]  = #copies with z > k, A1[k] = #allele-1 copies
-//                        with z > k (lane-private registers, carry-in adds);
+//   per-locus counters   TA[k] = #copies with z > k (low 16 bits) plus
+//                        #allele-1 copies with z > k (high 16 bits), one
+//                        lane-private register per k: a copy adds 1 or
+//                        0x10001 by its allele, two selects and one
+//                        three-input add per individual (a tile holds at most
+//                        65535 copies, so the halves never carry into each other);
 //   per-individual       popcount of the wave ballot of each predicate (SALU),
 //                        added into lane t of a collector register by that lane
 //                        alone, and into LDS once per 16 individuals.
@@ -140,11 +143,10 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
             P1[k] = P[((size_t)l * K + k) * A + 1];
         }
     }
-    uint32_t T[K], A1[K];   // entries 0..K-2 used
+    uint32_t TA[K];   // entries 0..K-2 used
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        T[k] = 0;
-        A1[k] = 0;
+        TA[k] = 0;
     }
     uint32_t Ttot = 0, A1tot = 0;
     LikeAcc acc;
@@ -186,6 +188,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                 const bool valid = code != 3u;
                 const bool x0 = code == 2u;
                 const bool x1 = code != 0u;
+                const uint32_t X0 = x0 ? 0x10001u : 1u, X1 = x1 ? 0x10001u : 1u;
                 const uint32_t wa = h ? w.z : w.x;
                 const uint32_t wb = h ? w.w : w.y;
                 const uint64_t vm = ballot(valid);
@@ -199,10 +202,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                     for (int k = 0; k < K - 1; k++) {
                         const bool m0 = valid && u0 > (uint32_t)k;
                         const bool m1 = valid && u1 > (uint32_t)k;
-                        T[k] += m0;
-                        T[k] += m1;
-                        A1[k] += m0 && x0;
-                        A1[k] += m1 && x1;
+                        TA[k] += (m0 ? X0 : 0u) + (m1 ? X1 : 0u);
                         nk[k + 1] = (uint32_t)(__popcll(ballot(m0)) + __popcll(ballot(m1)));
                     }
                 } else {
@@ -241,8 +241,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                         const uint64_t a1 = ballot(pre0[k] < r1), b1 = ballot(pre1[k] < r1);
                         const uint64_t m0 = (xm0 & b0) | (~xm0 & a0);   // z0 > k
                         const uint64_t m1 = (xm1 & b1) | (~xm1 & a1);   // z1 > k
-                        T[k] = add_lane_bit(add_lane_bit(T[k], m0), m1);
-                        A1[k] = add_lane_bit(add_lane_bit(A1[k], m0 & xm0), m1 & xm1);
+                        TA[k] += lane_select(m0, X0) + lane_select(m1, X1);
                         nk[k + 1] = (uint32_t)(__popcll(m0) + __popcll(m1));
                         if (ZOUT) {
                             z0 += (int)((m0 >> lane) & 1u);
@@ -283,8 +282,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
         uint32_t prevT = Ttot, prevA = A1tot;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const uint32_t tk = (k < K - 1) ? T[k] : 0u;
-            const uint32_t ak = (k < K - 1) ? A1[k] : 0u;
+            const uint32_t tk = (k < K - 1) ? (TA[k] & 0xFFFFu) : 0u;
+            const uint32_t ak = (k < K - 1) ? (TA[k] >> 16) : 0u;
             const uint32_t n1 = prevA - ak;
             const uint32_t nt = prevT - tk;
             dst[k * A + 0] = (uint16_t)(nt - n1);
@@ -320,7 +319,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
 // Byte-per-copy path (any allele count), K templated.  One lane per genotype
 // word row, i.e. per pair of individuals (2h, 2h+1), 256 rows per block; the
 // block walks a tile of loci.  Per locus the K x A allele frequencies are
-// wave-uniform, so they are staged in LDS a chunk of loci at a time, and the
+// wave-uniform, so they are staged in LDS a chunk of loci at a time (allele
+// major: a copy reads its allele's K frequencies at constant offsets), and the
 // locus's allele-by-population histogram is a set of LDS bins every lane adds
 // into.  Per-individual counts stay in lane registers as "z > k" counts, as in
 // the 2-bit path.  One Philox block per (locus, pair): words 0,1 serve the two
@@ -349,11 +349,16 @@ __device__ __forceinline__ void byte_copy(int x, bool valid, uint32_t w, const d
             z += gt;
         }
     } else {
+        // the copy's allele row of the staged frequencies: K doubles, constant offsets
+        const double *px = s_pl + x * K;
         double pre[K];
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            s = s + q[k] * s_pl[k * A + x];   // product and running sum in the reference's order
+            // product and running sum in the reference's order; the sum starts at
+            // 0.0, and 0.0 + c == c for the non-negative products here
+            const double c = q[k] * px[k];
+            s = k ? s + c : c;
             pre[k] = s;
         }
         if (LIKE) {
@@ -391,7 +396,7 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
     SGPU_G double *likepart = g(C.likepart);
     SGPU_G int8_t *zout = g(C.zout);
     extern __shared__ double s_dyn[];
-    double *s_p = s_dyn;                                  // [chunk][K][A] frequencies
+    double *s_p = s_dyn;                                  // [chunk][A][K] frequencies
     uint32_t *s_h = (uint32_t *)(s_dyn + (size_t)chunk * K * A);   // [chunk][K][A] counts
 
     const int R = (int)blockDim.x;                // word rows per block (64, 128 or 256)
@@ -418,10 +423,18 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
     for (int c0 = lt0; c0 < lt1; c0 += chunk) {
         const int nc = min(chunk, lt1 - c0);
         __syncthreads();
-        for (int t = threadIdx.x; t < nc * KA; t += R) {
-            if (!INIT) {
-                s_p[t] = P[(size_t)c0 * KA + t];
+        if (!INIT) {
+            // P[l][k][a] -> s_p[l - c0][a][k], one (locus, allele) row of K per thread
+            for (int r = threadIdx.x; r < nc * A; r += R) {
+                const int j = r / A, a = r - j * A;
+                const SGPU_G double *src = P + (size_t)(c0 + j) * KA + a;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    s_p[r * K + k] = src[k * A];
+                }
             }
+        }
+        for (int t = threadIdx.x; t < nc * KA; t += R) {
             s_h[t] = 0u;
         }
         __syncthreads();
