@@ -37,9 +37,10 @@ enum {
      * bits 2j..2j+1.  Lpad = L rounded up to 256; padding codes are 3. */
     SGPU_GENO_SNP2 = 1,
     /* Any allele count, one byte per allele copy (255 = missing).  Words are
-     * uint32 laid out [ceil(N/2)][Lpad]: word (h, l) holds bytes (ind 2h copy 0,
-     * ind 2h copy 1, ind 2h+1 copy 0, ind 2h+1 copy 1), low byte first; Lpad as
-     * above.  Allele indices are < 127 (int8 input). */
+     * uint32 laid out [L][Gpad], locus-major: word (h, l) at l * Gpad + h holds
+     * bytes (ind 2h copy 0, ind 2h copy 1, ind 2h+1 copy 0, ind 2h+1 copy 1), low
+     * byte first.  Gpad = ceil(N/2) rounded up to 64; padding words are all 255.
+     * Allele indices are < 127 (int8 input). */
     SGPU_GENO_BYTE = 2
 };
 
@@ -116,10 +117,12 @@ int sgpu_create(const sgpu_model *m, const sgpu_data *d, sgpu_ctx **out);
  *                 individuals per block; default 4096)
  *   metro_blocks  blocks per Metropolis-ratio launch (sets the loci per block;
  *                 default 1024)
- *   byte_loci     loci per byte-format UpdateZ block (default 64 below 16384 loci,
- *                 else 256; at most 256)
+ *   byte_loci     loci per byte-format UpdateZ block (default: the longest of 64,
+ *                 32, 16 that gives z_blocks blocks over the batch, else 16 to 64
+ *                 by K; at most 256)
  *   byte_chunk    loci whose K x A frequencies and counts are staged in LDS at
- *                 once by the byte-format UpdateZ (default: as many as fit 60 KiB)
+ *                 once by the byte-format UpdateZ (default and upper bound: 16
+ *                 loci within 32 KiB)
  * The draws and counts do not depend on them; lnP(D) and the Metropolis
  * log-ratios are FP64 sums whose grouping does (differences in the last bits).
  * structure_amd/tune.py searches them per shape. */

@@ -490,7 +490,7 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__res
     const SGPU_G double *P = g(C.P);
     const SGPU_G double *qprop = g(C.qprop);
     SGPU_G double *delta_part = g(C.delta_part);
-    const int Lpad = C.Lpad, loci_per_range = C.loci_per_range;
+    const int Gpad = C.Gpad, loci_per_range = C.loci_per_range;
     const int i = bx * kNT + threadIdx.x;
     if (i >= m.N) {
         return;
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__res
     double pp = 1.0, pm = 1.0;
     int ep = 0, em = 0;
     for (int l = l0; l < l1; l++) {
-        const uint32_t word = geno[(size_t)(i >> 1) * Lpad + l];
+        const uint32_t word = geno[(size_t)l * Gpad + (i >> 1)];   // [L][Gpad]: lanes adjacent
         const int xa = (int)((word >> (16 * (i & 1))) & 0xffu);
         const int xb = (int)((word >> (16 * (i & 1) + 8)) & 0xffu);
         const SGPU_G double *Pl = P + (size_t)l * K * m.A;
@@ -1129,7 +1129,7 @@ __global__ __launch_bounds__(kNT) void k_init_eps(const ChainDesc *__restrict__ 
     } else {
         const int G = (m.N + 1) / 2;
         for (int g = 0; g < G; g++) {
-            const uint32_t word = geno[(size_t)g * Lpad + l];
+            const uint32_t word = geno[(size_t)l * C.Gpad + g];   // byte words are [L][Gpad]
             for (int c = 0; c < 4 && g * 2 + (c >> 1) < m.N; c++) {
                 const int x = (int)((word >> (8 * c)) & 0xffu);
                 if (x != 0xff) {

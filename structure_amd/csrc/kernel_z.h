@@ -385,7 +385,7 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
     const ChainDesc &C = cds[br.chain];
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const int bx = (int)br.bx, by = (int)br.by;
-    const int N = C.m.N, L = C.m.L, A = C.m.A, Lpad = C.Lpad, G = C.G;
+    const int N = C.m.N, L = C.m.L, A = C.m.A, Gpad = C.Gpad, G = C.G;
     const int zt = C.zt_loci, chunk = C.z_chunk, n_lt = C.n_lt;
     const uint32_t seed = C.m.seed, chain = C.m.chain;
     const SGPU_G uint32_t *geno = g(C.geno);
@@ -438,9 +438,13 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
             s_h[t] = 0u;
         }
         __syncthreads();
+        // words [L][Gpad]: the lanes' pairs are adjacent words of one locus; the
+        // next locus's word is loaded one iteration ahead
+        uint32_t nextw = rowok ? geno[(size_t)c0 * Gpad + h] : 0xffffffffu;
         for (int j = 0; j < nc; j++) {
             const int l = c0 + j;
-            const uint32_t word = rowok ? geno[(size_t)h * Lpad + l] : 0xffffffffu;
+            const uint32_t word = nextw;
+            nextw = (rowok && j + 1 < nc) ? geno[(size_t)(l + 1) * Gpad + h] : 0xffffffffu;
             if (word != 0xffffffffu) {
                 const u32x4 w = philox4x32_10(rep, (uint32_t)l, (uint32_t)h,
                                               (INIT ? PH_INITZ : PH_Z) << 24, seed, chain);

@@ -65,8 +65,12 @@ constexpr int kMetroBlocksDefault = 1024;
 constexpr size_t kZLdsMax = 48 * 1024;
 // Most loci one Metropolis-ratio block walks.
 constexpr long long kMetroMaxLoci = 2048;
-// LDS per byte-format UpdateZ block for the staged frequencies and counts.
-constexpr size_t kByteLdsMax = 60 * 1024;
+// LDS per byte-format UpdateZ block for the staged frequencies and counts, and
+// the most loci staged at once: small enough that five or more blocks share a
+// CU (measured on 32 microsatellite data sets with K x A up to 160: chunks of
+// up to 60 KiB ran 1.15x slower than chunks of at most 16 loci and 32 KiB).
+constexpr size_t kByteLdsMax = 32 * 1024;
+constexpr int kByteChunkMax = 16;
 
 // One launch over a set of chains: the block map and its LDS need.
 struct Plan {
@@ -82,7 +86,7 @@ struct KGroup {
 };
 
 struct Dataset {
-    int fmt = 0, N = 0, L = 0, A = 0, Lpad = 0, G = 0;
+    int fmt = 0, N = 0, L = 0, A = 0, Lpad = 0, G = 0, Gpad = 0;
     uint32_t *geno = nullptr;
     int32_t *nall = nullptr, *ahist = nullptr;
     int nhist = 0, nloci_nonzero = 0;
@@ -537,11 +541,14 @@ int check_data(const sgpu_data *d, int N, int L, int A, int di)
                         d->nall[l], A);
         }
     }
-    const size_t lp = need / cdiv(N, d->format == SGPU_GENO_SNP2 ? 16 : 2);
-    for (size_t row = 0; row * lp < need; row++) {
-        const uint32_t *w = d->geno + row * lp;
+    // 2-bit words are [rows][Lpad], byte words [L][Gpad]: (row, l) at
+    // row * row_stride + l * locus_stride
+    const size_t rows = d->format == SGPU_GENO_SNP2 ? cdiv(N, 16) : need / L;
+    const size_t row_stride = d->format == SGPU_GENO_SNP2 ? need / rows : 1;
+    const size_t locus_stride = d->format == SGPU_GENO_SNP2 ? 1 : rows;
+    for (size_t row = 0; row < rows; row++) {
         for (int l = 0; l < L; l++) {
-            const uint32_t x = w[l];
+            const uint32_t x = d->geno[row * row_stride + (size_t)l * locus_stride];
             if (d->format == SGPU_GENO_SNP2) {
                 for (int j = 0; j < 16; j++) {
                     const uint32_t code = (x >> (2 * j)) & 3u;
@@ -635,7 +642,7 @@ BatchTiles batch_tiles(const sgpu_model *models, const sgpu_data *data,
     BatchTiles t{};
     t.snp_rows = (int)std::max<long long>(1, snp_work / std::max(z_target, 1));
     // byte format: the largest block (word rows) that launches at most 15% more
-    // lanes than 64-row blocks would, then the longest locus tile that still
+    // lanes than 64-row blocks would, then the longest locus tile (<= 64) that still
     // gives z_target blocks; failing that, the shortest tile whose per-tile
     // ancestry-count partials (N K int32 per tile) stay within half of the
     // genotype stream (2 N L bytes), i.e. at least 4 K loci
@@ -671,14 +678,14 @@ BatchTiles batch_tiles(const sgpu_model *models, const sgpu_data *data,
         }
     }
     t.byte_zt = 0;
-    for (int zt : {256, 128, 64, 32, 16}) {
+    for (int zt : {64, 32, 16}) {
         if (blocks(zt) >= z_target) {
             t.byte_zt = zt;
             break;
         }
     }
     if (t.byte_zt == 0) {
-        t.byte_zt = 256;
+        t.byte_zt = 64;
         while (t.byte_zt > 16 && t.byte_zt / 2 >= 4 * kmax) {
             t.byte_zt /= 2;
         }
@@ -700,6 +707,7 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
     d.fmt = ds.fmt;
     d.Lpad = ds.Lpad;
     d.G = ds.G;
+    d.Gpad = ds.Gpad;
     d.geno = ds.geno;
     d.nall = ds.nall;
     d.ahist = ds.ahist;
@@ -729,8 +737,8 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         d.rows_per_tile = bt.byte_rows;
         d.n_it = (int)cdiv(d.G, bt.byte_rows);
         const size_t per_locus = (size_t)K * A * (sizeof(double) + sizeof(uint32_t));
-        size_t chunk = std::max<size_t>(1, std::min<size_t>(kByteLdsMax / per_locus,
-                                                            (size_t)d.zt_loci));
+        size_t chunk = std::max<size_t>(
+            1, std::min<size_t>({kByteLdsMax / per_locus, (size_t)d.zt_loci, (size_t)kByteChunkMax}));
         if (bt.byte_chunk > 0) {
             chunk = std::min<size_t>(chunk, (size_t)bt.byte_chunk);
         }
@@ -984,6 +992,7 @@ int sgpu_create_batch(const sgpu_model *models, int32_t nchains, const sgpu_data
         ds.A = m.A;
         ds.G = (int)cdiv(m.N, ds.fmt == SGPU_GENO_SNP2 ? 16 : 2);
         ds.Lpad = (int)cdiv(m.L, kZLoci) * kZLoci;
+        ds.Gpad = (int)(sgpu_packed_words(ds.fmt, m.N, m.L) / (size_t)m.L);   // byte format
         if ((rc = upload_dataset(c, ds, &data[di]))) {
             return cleanup(rc);
         }

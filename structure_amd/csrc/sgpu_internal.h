@@ -44,13 +44,14 @@ struct ModelArgs {
 struct ChainDesc {
     ModelArgs m;
     int fmt;                       // 1 = 2-bit SNP words, 2 = byte per copy
-    int Lpad, G;                   // padded loci per word row, word rows
+    int Lpad, G;                   // padded loci per word row (2-bit), word rows
+    int Gpad;                      // byte format: padded word rows, the stride between loci
     int n_lt, n_it, rows_per_tile; // UpdateZ tiles: loci tiles, individual tiles, rows per tile
     int zt_loci, z_chunk;          // loci per UpdateZ block; byte path: loci staged in LDS at once
     int q_blocks, l_blocks;        // cdiv(N, 256), cdiv(L, 256)
     int n_ranges, loci_per_range;  // Metropolis locus ranges
     int nhist, nloci_nonzero;      // allele-count histogram size, loci with alleles
-    const uint32_t *geno;          // packed words [G][Lpad]
+    const uint32_t *geno;          // packed words: 2-bit [G][Lpad], byte [L][Gpad]
     const int32_t *nall;           // [L]
     const int32_t *ahist;          // (allele count, loci) pairs
     double *Q, *P, *eps, *F, *alpha, *lambda;

@@ -3,12 +3,13 @@
  *
  * Input is the reference's recoded genotype array Geno[ind][line][loc]
  * (structure.h:22 GenPos; alleles recoded 0..k-1 by first appearance,
- * datain.c:824-904; -9 = missing).  Two device formats:
- *   SGPU_GENO_SNP2  2 bits per (ind, locus), 16 individuals per uint32 word
- *   SGPU_GENO_BYTE  1 byte per allele copy, 2 individuals per uint32 word
- * Both are laid out [word row][Lpad] so that consecutive loci are
- * consecutive words: the UpdateZ kernel maps one locus to one lane and
- * streams a row with fully coalesced 256-byte wavefront loads.
+ * datain.c:824-904; -9 = missing).  Two device formats, each laid out so that
+ * the lanes of a wavefront read consecutive words (256-byte loads):
+ *   SGPU_GENO_SNP2  2 bits per (ind, locus), 16 individuals per uint32 word,
+ *                   [word row][Lpad]: its UpdateZ kernel maps one locus to a lane
+ *   SGPU_GENO_BYTE  1 byte per allele copy, 2 individuals per uint32 word,
+ *                   [locus][Gpad]: its kernels map one individual pair (or one
+ *                   individual) to a lane
  */
 #include <stdint.h>
 #include <string.h>
@@ -17,7 +18,10 @@
 
 #define ZLOCI 256   /* loci per UpdateZ block; rows are padded to a multiple */
 
+#define WAVE 64    /* byte format: word rows (individual pairs) padded to a wavefront */
+
 static size_t lpad_of(int32_t L) { return ((size_t)L + ZLOCI - 1) / ZLOCI * ZLOCI; }
+static size_t gpad_of(int32_t N) { return (((size_t)N + 1) / 2 + WAVE - 1) / WAVE * WAVE; }
 
 size_t sgpu_packed_words(int32_t fmt, int32_t N, int32_t L)
 {
@@ -28,7 +32,7 @@ size_t sgpu_packed_words(int32_t fmt, int32_t N, int32_t L)
         return ((size_t)N + 15) / 16 * lpad_of(L);
     }
     if (fmt == SGPU_GENO_BYTE) {
-        return ((size_t)N + 1) / 2 * lpad_of(L);
+        return (size_t)L * gpad_of(N);
     }
     return 0;
 }
@@ -69,28 +73,33 @@ int sgpu_pack_snp2(const int8_t *geno, int32_t N, int32_t L, uint32_t *out, size
 
 int sgpu_pack_byte(const int8_t *geno, int32_t N, int32_t L, uint32_t *out, size_t out_words)
 {
-    const size_t lp = lpad_of(L);
+    const size_t gp = gpad_of(N);
     if (!geno || !out || out_words != sgpu_packed_words(SGPU_GENO_BYTE, N, L)) {
         return SGPU_ERR_ARG;
     }
-    memset(out, 0xff, out_words * sizeof(uint32_t));
-    for (int32_t i = 0; i < N; i++) {
-        uint32_t *row = out + (size_t)(i >> 1) * lp;
-        const int sh = 16 * (i & 1);
-        for (int c = 0; c < 2; c++) {
-            const int8_t *g = geno + ((size_t)i * 2 + c) * L;
-            for (int32_t l = 0; l < L; l++) {
-                const int x = g[l];
-                uint32_t v;
-                if (x == -9) {
-                    v = 0xffu;
-                } else if (x >= 0 && x < 255) {
-                    v = (uint32_t)x;
-                } else {
-                    return SGPU_ERR_ARG;
-                }
+    memset(out, 0xff, out_words * sizeof(uint32_t));   /* padding rows: all copies missing */
+    /* loci in blocks of 64, so that both the reads along a copy and the writes
+     * down a locus column stay within a few KiB */
+    for (int32_t l0 = 0; l0 < L; l0 += 64) {
+        const int32_t l1 = l0 + 64 < L ? l0 + 64 : L;
+        for (int32_t i = 0; i < N; i++) {
+            const int sh = 16 * (i & 1);
+            for (int c = 0; c < 2; c++) {
+                const int8_t *g = geno + ((size_t)i * 2 + c) * L;
                 const int s = sh + 8 * c;
-                row[l] = (row[l] & ~(0xffu << s)) | (v << s);
+                for (int32_t l = l0; l < l1; l++) {
+                    const int x = g[l];
+                    uint32_t v;
+                    if (x == -9) {
+                        v = 0xffu;
+                    } else if (x >= 0 && x < 255) {
+                        v = (uint32_t)x;
+                    } else {
+                        return SGPU_ERR_ARG;
+                    }
+                    uint32_t *w = out + (size_t)l * gp + (size_t)(i >> 1);
+                    *w = (*w & ~(0xffu << s)) | (v << s);
+                }
             }
         }
     }

@@ -27,7 +27,7 @@ def candidates(models, datasets, data_of_chain):
     fmts = {datasets[j][0] for j in data_of_chain}
     zb = [1536, 2048, 3072, 4096, 5120, 6144, 8192] if GENO_SNP2 in fmts else [0]
     mb = [512, 1024, 2048]
-    bl = [32, 64, 128, 256] if GENO_BYTE in fmts else [0]
+    bl = [16, 32, 64] if GENO_BYTE in fmts else [0]
     for z, m, b in itertools.product(zb, mb, bl):
         yield Tuning(z_blocks=z, metro_blocks=m, byte_loci=b, byte_chunk=0)
 

@@ -41,12 +41,11 @@ def _unpack_snp2(words, N, L):
 
 
 def _unpack_byte(words, N, L):
-    lpad = words.size // ((N + 1) // 2)
-    w = words.reshape(-1, lpad)[:, :L]
+    w = words.reshape(L, -1)   # [L][Gpad], locus-major
     out = np.empty((N, 2, L), np.int8)
     for i in range(N):
         for c in range(2):
-            b = (w[i >> 1] >> np.uint32(16 * (i & 1) + 8 * c)) & np.uint32(0xff)
+            b = (w[:, i >> 1] >> np.uint32(16 * (i & 1) + 8 * c)) & np.uint32(0xff)
             out[i, c] = np.where(b == 0xff, -9, b).astype(np.int8)
     return out
 
@@ -71,6 +70,10 @@ def test_pack_byte_roundtrip_keeps_copy_order():
     fmt, words = sa.pack(geno)
     assert fmt == sa.GENO_BYTE   # half-missing sites and (1,0) hets force the byte path
     assert np.array_equal(_unpack_byte(words, 21, 77), geno)
+    assert words.size == 77 * 64                      # 11 word rows padded to 64
+    w = words.reshape(77, 64)
+    assert np.all(w[:, 11:] == 0xffffffff)            # padded rows are missing
+    assert np.all(w[:, 10] >> np.uint32(16) == 0xffff)  # individual 21 of the last row missing
 
 
 @pytest.mark.parametrize("bad", [(1, 0), (0, -9), (-9, 1), (2, 2)])
