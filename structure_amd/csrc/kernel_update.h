@@ -79,7 +79,8 @@ __device__ __forceinline__ void wave_col_sums(const SGPU_G double *x, int n, int
 
 // ---------------------------------------------------------------------------
 // UpdateP (UpdateP.c:106-199): one thread per (locus, pop, allele), a block
-// holding whole (locus, pop) groups (A * (256 / A) threads).  Dirichlet
+// holding whole (locus, pop) groups: nall[l] threads each from the chain's work
+// list, or A threads each (256 / A groups) on the dense grid.  Dirichlet
 // parameters from the allele counts of the last UpdateZ (summed over the
 // individual-tile partials, coalesced), one gamma draw per thread, then the
 // group's sum in allele order from LDS and LogRDirichletDisc's rule
@@ -101,14 +102,26 @@ __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ 
     SGPU_G double *PSum = g(C.PSum);
     __shared__ double s_x[kNT];
     const int A = m.A;
-    const int gpb = kNT / A;   // (locus, pop) groups per block
-    const int grp = (int)threadIdx.x / A, aa = (int)threadIdx.x - grp * A;
-    const int idx = bx * gpb + grp;   // flat (locus, pop)
-    const bool live = grp < gpb && idx < m.L * m.K;
-    const int l = live ? idx / m.K : 0, k = live ? idx - l * m.K : 0;
+    int l, k, aa;
+    bool live;
+    if (C.pslots) {   // work list: a group's nall[l] threads are adjacent, allele order
+        const uint32_t e = g(C.pslots)[(size_t)bx * kNT + threadIdx.x];
+        live = e != 0xFFFFFFFFu;
+        l = live ? (int)(e >> 12) : 0;
+        k = live ? (int)((e >> 8) & 15u) : 0;
+        aa = (int)(e & 255u);
+    } else {          // dense grid: A threads per (locus, pop) group
+        const int gpb = kNT / A;   // groups per block
+        const int grp = (int)threadIdx.x / A;
+        const int idx = bx * gpb + grp;   // flat (locus, pop)
+        aa = (int)threadIdx.x - grp * A;
+        live = grp < gpb && idx < m.L * m.K;
+        l = live ? idx / m.K : 0;
+        k = live ? idx - l * m.K : 0;
+    }
     const int na = live ? nall[l] : 0;
     const bool act = aa < na;
-    const size_t pos = (size_t)idx * A + aa;
+    const size_t pos = ((size_t)l * m.K + k) * A + aa;
     double x = 0.0;
     if (act) {
         // the partials are independent loads: eight in flight, exact integer sums

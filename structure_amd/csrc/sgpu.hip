@@ -87,6 +87,7 @@ struct KGroup {
 
 struct Dataset {
     int fmt = 0, N = 0, L = 0, A = 0, Lpad = 0, G = 0, Gpad = 0;
+    std::vector<int32_t> nall_host;   // NumAlleles, for the UpdateP work lists
     uint32_t *geno = nullptr;
     int32_t *nall = nullptr, *ahist = nullptr;
     int nhist = 0, nloci_nonzero = 0;
@@ -600,6 +601,7 @@ int upload_dataset(sgpu_ctx *c, Dataset &ds, const sgpu_data *d)
     }
     ds.nhist = (int)hist.size();
     ds.nloci_nonzero = ds.L - nzero;
+    ds.nall_host.assign(d->nall, d->nall + ds.L);
     HIP_OK(hipMemcpyAsync(ds.geno, d->geno, need * sizeof(uint32_t), hipMemcpyHostToDevice,
                           c->stream));
     HIP_OK(hipMemcpyAsync(ds.nall, d->nall, (size_t)ds.L * sizeof(int32_t), hipMemcpyHostToDevice,
@@ -698,6 +700,56 @@ BatchTiles batch_tiles(const sgpu_model *models, const sgpu_data *data,
     return t;
 }
 
+// UpdateP work list of a chain: the dense grid gives every (locus, pop) group A
+// threads and leaves a - nall[l] of them idle, which with microsatellites
+// (A the largest allele count) is most of a launch.  The list gives a group
+// exactly nall[l] threads, groups packed whole into blocks of kNT.  Used when it
+// saves at least an eighth of the threads.
+int build_p_slots(sgpu_ctx *c, HostChain &h, const Dataset &ds)
+{
+    ChainDesc &d = h.d;
+    const int L = h.m.L, K = h.m.K, A = h.m.A;
+    const int gpb = kNT / A;
+    d.p_blocks = (int)cdiv((size_t)L * K, (size_t)gpb);
+    d.pslots = nullptr;
+    if (L >= (1 << 20) || A > 255) {
+        return SGPU_OK;   // the fields of a list entry
+    }
+    std::vector<uint32_t> v;
+    int fill = 0;
+    for (int l = 0; l < L; l++) {
+        const int na = ds.nall_host[l];
+        for (int k = 0; k < K && na > 0; k++) {
+            if (fill + na > kNT) {
+                v.insert(v.end(), (size_t)(kNT - fill), 0xFFFFFFFFu);
+                fill = 0;
+            }
+            for (int a = 0; a < na; a++) {
+                v.push_back((uint32_t)l << 12 | (uint32_t)k << 8 | (uint32_t)a);
+            }
+            fill += na;
+        }
+    }
+    if (v.empty() || fill > 0) {
+        v.insert(v.end(), (size_t)(kNT - fill), 0xFFFFFFFFu);
+    }
+    const size_t blocks = v.size() / kNT;
+    if (blocks * 8 > (size_t)d.p_blocks * 7) {
+        return SGPU_OK;   // the dense grid is nearly as small
+    }
+    uint32_t *dev = nullptr;
+    int rc = dalloc(c, &dev, v.size());
+    if (rc) {
+        return rc;
+    }
+    HIP_OK(hipMemcpyAsync(dev, v.data(), v.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));   // v is freed on return
+    d.pslots = dev;
+    d.p_blocks = (int)blocks;
+    return SGPU_OK;
+}
+
 int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &bt)
 {
     const sgpu_model &m = h.m;
@@ -775,7 +827,7 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         (rc = dalloc(c, &d.like3, 3))) {
         return rc;
     }
-    return SGPU_OK;
+    return build_p_slots(c, h, ds);
 }
 
 int upload_descriptors(sgpu_ctx *c)
@@ -797,10 +849,7 @@ int build_plans(sgpu_ctx *c)
     auto qb = [](const HostChain &h) { return std::make_pair(h.d.q_blocks, 1); };
     auto lb = [](const HostChain &h) { return std::make_pair(h.d.l_blocks, 1); };
     if ((rc = build_plan(c, c->p_update_p, all_chains,
-                         [](const HostChain &h) {
-                             return std::make_pair(
-                                 (int)cdiv((size_t)h.m.L * h.m.K, (size_t)(kNT / h.m.A)), 1);
-                         })) ||
+                         [](const HostChain &h) { return std::make_pair(h.d.p_blocks, 1); })) ||
         (rc = build_plan(c, c->p_fold, all_chains,
                          [](const HostChain &h) {
                              return std::make_pair((int)cdiv((size_t)h.m.N * h.m.K, kNT), kRowSplit);

@@ -51,6 +51,11 @@ struct ChainDesc {
     int q_blocks, l_blocks;        // cdiv(N, 256), cdiv(L, 256)
     int n_ranges, loci_per_range;  // Metropolis locus ranges
     int nhist, nloci_nonzero;      // allele-count histogram size, loci with alleles
+    int p_blocks;                  // UpdateP blocks of this chain
+    // UpdateP work list [p_blocks][256]: (l << 12 | k << 8 | a) for a < nall[l],
+    // each (locus, pop) group whole within one block, ~0 = idle thread; null:
+    // the dense (locus, pop, allele < A) grid
+    const uint32_t *pslots;
     const uint32_t *geno;          // packed words: 2-bit [G][Lpad], byte [L][Gpad]
     const int32_t *nall;           // [L]
     const int32_t *ahist;          // (allele count, loci) pairs
