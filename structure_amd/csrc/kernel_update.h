@@ -490,7 +490,13 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__res
     }
 }
 
-// Byte-per-copy path (any allele count), K templated.
+// Byte-per-copy path (any allele count), K templated.  The frequencies of a
+// chunk of loci are staged in LDS allele-major ([locus][allele][pop]): a copy
+// reads its allele's K values at constant offsets, and the block reads each
+// frequency from global memory once instead of once per lane.
+constexpr int kMetroByteLds = 4096;   // doubles of staged frequencies (32 KiB)
+static_assert(kMetroByteLds >= kMaxAlleles * kMaxK, "one locus must fit the staging buffer");
+
 template <int K>
 __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__restrict__ cds,
                                                            const BlockRef *__restrict__ map,
@@ -503,53 +509,77 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__res
     const SGPU_G double *P = g(C.P);
     const SGPU_G double *qprop = g(C.qprop);
     SGPU_G double *delta_part = g(C.delta_part);
-    const int Gpad = C.Gpad, loci_per_range = C.loci_per_range;
+    const int Gpad = C.Gpad, loci_per_range = C.loci_per_range, A = m.A;
     const int i = bx * kNT + threadIdx.x;
-    if (i >= m.N) {
-        return;
-    }
+    const bool active = i < m.N;
     double tq[K], cq[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        tq[k] = qprop[(size_t)i * K + k];
-        cq[k] = Q[(size_t)i * K + k];
+        tq[k] = active ? qprop[(size_t)i * K + k] : 1.0;
+        cq[k] = active ? Q[(size_t)i * K + k] : 1.0;
     }
+    __shared__ double s_p[kMetroByteLds];
+    const int chunk = kMetroByteLds / (A * K);   // >= 1 (A <= kMaxAlleles)
     const int l0 = by * loci_per_range;
     const int l1 = min(l0 + loci_per_range, m.L);
+    const int sh = 16 * (i & 1);
     double pp = 1.0, pm = 1.0;
     int ep = 0, em = 0;
-    for (int l = l0; l < l1; l++) {
-        const uint32_t word = geno[(size_t)l * Gpad + (i >> 1)];   // [L][Gpad]: lanes adjacent
-        const int xa = (int)((word >> (16 * (i & 1))) & 0xffu);
-        const int xb = (int)((word >> (16 * (i & 1) + 8)) & 0xffu);
-        const SGPU_G double *Pl = P + (size_t)l * K * m.A;
-        if (xa != 0xff) {
-            double tp = 0.0, tm = 0.0;
+    for (int c0 = l0; c0 < l1; c0 += chunk) {
+        const int nc = min(chunk, l1 - c0);
+        __syncthreads();
+        // P[l][k][a] -> s_p[l - c0][a][k], one (locus, allele) row of K per thread
+        for (int r = threadIdx.x; r < nc * A; r += kNT) {
+            const int j = r / A, a = r - j * A;
+            const SGPU_G double *src = P + (size_t)(c0 + j) * K * A + a;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                tp = fma(tq[k], Pl[k * m.A + xa], tp);
-                tm = fma(cq[k], Pl[k * m.A + xa], tm);
+                s_p[r * K + k] = src[k * A];
             }
-            pp *= tp;
-            pm *= tm;
         }
-        if (xb != 0xff) {
-            double tp = 0.0, tm = 0.0;
+        __syncthreads();
+        if (!active) {
+            continue;
+        }
+        for (int l = c0; l < c0 + nc; l++) {
+            // words [L][Gpad]: lanes 2j and 2j+1 read the same word, adjacent lanes
+            // adjacent words
+            const uint32_t word = geno[(size_t)l * Gpad + (i >> 1)];
+            const int xa = (int)((word >> sh) & 0xffu);
+            const int xb = (int)((word >> (sh + 8)) & 0xffu);
+            const double *pl = s_p + (size_t)(l - c0) * A * K;
+            if (xa != 0xff) {
+                const double *px = pl + xa * K;
+                double tp = 0.0, tm = 0.0;
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                tp = fma(tq[k], Pl[k * m.A + xb], tp);
-                tm = fma(cq[k], Pl[k * m.A + xb], tm);
+                for (int k = 0; k < K; k++) {
+                    tp = fma(tq[k], px[k], tp);
+                    tm = fma(cq[k], px[k], tm);
+                }
+                pp *= tp;
+                pm *= tm;
             }
-            pp *= tp;
-            pm *= tm;
+            if (xb != 0xff) {
+                const double *px = pl + xb * K;
+                double tp = 0.0, tm = 0.0;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    tp = fma(tq[k], px[k], tp);
+                    tm = fma(cq[k], px[k], tm);
+                }
+                pp *= tp;
+                pm *= tm;
+            }
+            int e;
+            pp = frexp(pp, &e);
+            ep += e;
+            pm = frexp(pm, &e);
+            em += e;
         }
-        int e;
-        pp = frexp(pp, &e);
-        ep += e;
-        pm = frexp(pm, &e);
-        em += e;
     }
-    delta_part[(size_t)by * m.N + i] = (log_d(pp) - log_d(pm)) + (double)(ep - em) * kLn2;
+    if (active) {
+        delta_part[(size_t)by * m.N + i] = (log_d(pp) - log_d(pm)) + (double)(ep - em) * kLn2;
+    }
 }
 
 __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restrict__ cds,
