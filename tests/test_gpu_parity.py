@@ -281,11 +281,10 @@ TINY = [  # (format, N, L, K): single individuals and loci, odd N, L off every t
 ]
 
 
-@pytest.mark.parametrize("fmt,N,L,K", TINY)
-def test_tiny_and_odd_shapes_follow_the_oracle(fmt, N, L, K):
-    """Edge shapes through whole sweeps (Gibbs and Metropolis Q, F model, alpha
-    and lambda) with graph replay: the same trajectory as the oracle."""
-    rng = np.random.default_rng(N * 1000 + L * 10 + K)
+def _follow_oracle(fmt, N, L, K, A_byte=5, sweeps=7):
+    """Whole sweeps (Gibbs and Metropolis Q, F model, alpha and lambda) with
+    graph replay on a small shape: the same trajectory as the oracle."""
+    rng = np.random.default_rng(N * 1000 + L * 10 + K + A_byte)
     if fmt == "snp":   # 2-bit words: biallelic, both copies missing together
         geno, nall = random_geno(rng, N, L, 2, miss=0.0, biallelic=True)
         geno = np.sort(geno, axis=1)   # heterozygotes as (0, 1)
@@ -294,15 +293,15 @@ def test_tiny_and_odd_shapes_follow_the_oracle(fmt, N, L, K):
             geno[:, c, :] = np.where(gone, -9, geno[:, c, :])
         A, f = 2, sa.GENO_SNP2
     else:
-        geno, nall = random_geno(rng, N, L, 5, miss=0.1)
-        A, f = 5, sa.GENO_BYTE
+        geno, nall = random_geno(rng, N, L, A_byte, miss=0.1)
+        A, f = A_byte, sa.GENO_BYTE
     kw = dict(freqscorr=1, inferalpha=1, inferlambda=1, burnin=3, metrofreq=2)
     och, gch = _pair(geno, nall, K, A, fmt=f, seed=99, chain=K, **kw)
     assert gch.fmt == f
     gch.init()
     och.init()
-    gch.run(0, 7)
-    for rep in range(7):
+    gch.run(0, sweeps)
+    for rep in range(sweeps):
         och.sweep(rep)
     assert np.array_equal(gch.get(sa.NA), och.nA)
     assert np.array_equal(gch.get(sa.NQ), och.nQ)
@@ -311,6 +310,19 @@ def test_tiny_and_odd_shapes_follow_the_oracle(fmt, N, L, K):
                      (sa.PSUM, och.PSum)]:
         _close(gch.get(sel), ref, 1e-9)
     _close(gch.get(sa.LIKE), [och.like, och.sumlikes, och.sumsqlikes], 1e-9)
+
+
+@pytest.mark.parametrize("fmt,N,L,K", TINY)
+def test_tiny_and_odd_shapes_follow_the_oracle(fmt, N, L, K):
+    _follow_oracle(fmt, N, L, K)
+
+
+@pytest.mark.parametrize("N,L,K,A", [(9, 11, 16, 120), (40, 6, 3, 127), (130, 3, 8, 64)])
+def test_many_alleles_follow_the_oracle(N, L, K, A):
+    """MAXALLELES up to 127 (the int8 input's largest allele index + 1): one locus
+    per LDS chunk in UpdateZ, two in the Metropolis ratio, one or two (locus, pop)
+    groups per UpdateP block."""
+    _follow_oracle("byte", N, L, K, A_byte=A)
 
 
 @pytest.mark.parametrize("variant", ["full", "no_tail_in_burnin"])
