@@ -5,8 +5,8 @@
 // optionally, the lnP(D) terms of CalcLike (CalcLike.c:75-133).  Z itself is
 // never written to HBM (a debug copy can be requested).
 //
-// Mapping: one locus per lane, 256 loci per block (4 wavefronts); a block
-// walks the individuals of its individual tile.  Each lane keeps its locus's
+// Mapping of the 2-bit kernel: one locus per lane, 256 loci per block (4
+// wavefronts); a block walks the individuals of its individual tile.  Each lane keeps its locus's
 // P column and allele-by-population counters in registers, so the per-locus
 // histogram needs no atomics; Q[i,:] is wave-uniform (scalar loads); the
 // per-individual counts are wave ballots reduced in LDS.
@@ -317,8 +317,9 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
 
 // ---------------------------------------------------------------------------
 // Byte-per-copy path (any allele count), K templated.  One lane per genotype
-// word row, i.e. per pair of individuals (2h, 2h+1), 256 rows per block; the
-// block walks a tile of loci.  Per locus the K x A allele frequencies are
+// word row, i.e. per pair of individuals (2h, 2h+1), 64 to 256 rows per block;
+// the block walks a tile of loci, and with the words stored locus-major a
+// wavefront's rows are adjacent words of one locus.  Per locus the K x A allele frequencies are
 // wave-uniform, so they are staged in LDS a chunk of loci at a time (allele
 // major: a copy reads its allele's K frequencies at constant offsets), and the
 // locus's allele-by-population histogram is a set of LDS bins every lane adds
