@@ -1014,7 +1014,6 @@ __global__ __launch_bounds__(kNT) void k_fst_finalize(const ChainDesc *__restric
 // Data-collection tail (output.c:50-101, UpdateSums :654-707): scalar sums
 // and lnP(D) from the UpdateZ partials (one block per chain, fixed order).
 // ---------------------------------------------------------------------------
-// lnP(D) of the last UpdateZ: the sum of its tile partials by one wave (lane 0).
 // lnP(D) of the last UpdateZ is the sum of its tile partials, split in two
 // halves summed by two waves (lane 0 of each), the halves then added in order.
 __device__ __forceinline__ double like_half(const ChainDesc &C, int half)
