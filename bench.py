@@ -105,9 +105,11 @@ def batched_ksweep(d, N, L, seed, sweeps, warmup, kmax=10, reps=10):
                 sum(N * L * m.K for m in models) * sweeps / t_batch, 1)}
 
 
-def _timed_run(ch, warmup, steps):
-    """Warm-up sweeps (burn-in), then `steps` timed sweeps from graph replay;
-    returns seconds."""
+def _timed_run(ch, warmup, steps, direct=False):
+    """Warm-up sweeps (burn-in), then `steps` timed sweeps from graph replay (or,
+    with `direct`, launched one kernel at a time); returns seconds."""
+    if direct:
+        ch.set_timing(True)   # timing mode launches every phase directly
     ch.init()
     ch.run(0, warmup)
     ch.synchronize()
@@ -204,12 +206,18 @@ def _ragged_datasets(seed, n=64):
     return specs, data
 
 
-def _batch_vs_plain(models, datasets, doc, warm, sweeps, plain_sweeps=None):
+def _r3(x):
+    return None if x is None else round(x, 3)
+
+
+def _batch_vs_plain(models, datasets, doc, warm, sweeps, plain_sweeps=None, direct=False):
     """Chain-sweeps/s of all chains in one batch against the same chains each
-    run alone (one after the other) on the GPU."""
+    run alone (one after the other) on the GPU.  With `direct` (for kernel
+    tracers) the batch's sweeps are launched without graphs and the chains are
+    not run alone."""
     import structure_amd as sa
     b = sa.Batch(models, datasets, doc)
-    t_batch = _timed_run(b, warm, sweeps)
+    t_batch = _timed_run(b, warm, sweeps, direct)
     # per-phase device time of a batch sweep (events around each phase, direct launches)
     nt = min(sweeps, 20)
     b.set_timing(True)
@@ -219,6 +227,9 @@ def _batch_vs_plain(models, datasets, doc, warm, sweeps, plain_sweeps=None):
     b.set_timing(False)
     phases = {p: round(v[0] / nt, 4) for p, v in tm.items()}
     b.close()
+    n = len(models)
+    if direct:
+        return n * sweeps / t_batch, None, phases
     ps = plain_sweeps or sweeps
     t_plain = 0.0
     for m, j in zip(models, doc):
@@ -226,7 +237,6 @@ def _batch_vs_plain(models, datasets, doc, warm, sweeps, plain_sweeps=None):
         ch = sa.Chain(m, fmt, words, nall)
         t_plain += _timed_run(ch, warm, ps)
         ch.close()
-    n = len(models)
     return n * sweeps / t_batch, n * ps / t_plain, phases
 
 
@@ -244,7 +254,7 @@ def workload_c5(args):
               for j, (sp, (_, _, _, A)) in enumerate(zip(specs, data))]
     dsets = [(f, w, na) for f, w, na, _ in data]
     batched, plain, phases = _batch_vs_plain(models, dsets, list(range(len(models))), warm,
-                                             sweeps)
+                                             sweeps, direct=args.direct)
     sites = sum(sp["N"] * sp["L"] * sp["K"] for sp in specs)
     return {"metric": "chain-sweeps/s, 64 ragged data sets in one batch",
             "value": round(batched, 3), "unit": "chain-sweeps/s", "n_gpus": 1, "steps": sweeps,
@@ -254,9 +264,10 @@ def workload_c5(args):
                                    "[1e3,2e5], half 2-bit SNP / half byte microsatellite "
                                    "(2-20 alleles), K in [2,8], F model + alpha",
                        "datasets": len(specs), "total_sites": sum(sp["N"] * sp["L"] for sp in specs)},
-            "plain_chain_sweeps_per_s": round(plain, 3), "speedup": round(batched / plain, 3),
+            "plain_chain_sweeps_per_s": _r3(plain), "speedup": _r3(plain and batched / plain),
             "site_updates_per_s": round(sites * batched / len(specs), 1),
-            "phase_ms_per_batch_sweep": phases}
+            "phase_ms_per_batch_sweep": phases,
+            "launch": "direct" if args.direct else "graph replay"}
 
 
 def workload_c1batch(args):
@@ -272,15 +283,17 @@ def workload_c1batch(args):
                             metrofreq=10, burnin=warm, numreps=sweeps, seed=args.seed + r,
                             chain=3) for r in range(100)]
     batched, plain, phases = _batch_vs_plain(models, [(fmt, words, d["nall"])], [0] * 100,
-                                             warm, sweeps, plain_sweeps=max(sweeps // 10, 20))
+                                             warm, sweeps, plain_sweeps=max(sweeps // 10, 20),
+                                             direct=args.direct)
     res = {"metric": "chain-sweeps/s, 100 replicate chains of config 1 in one batch",
            "value": round(batched, 3), "unit": "chain-sweeps/s", "n_gpus": 1, "steps": sweeps,
            "warmup": warm, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": f"synthetic microsatellites (seed {args.seed})",
            "config": {"workload": "config 1 x 100 seeds: N=200 L=500 A in [2,10] K=3, "
                                   "no F model, alpha inferred", "chains": 100},
-           "plain_chain_sweeps_per_s": round(plain, 3), "speedup": round(batched / plain, 3),
-           "phase_ms_per_batch_sweep": phases}
+           "plain_chain_sweeps_per_s": _r3(plain), "speedup": _r3(plain and batched / plain),
+           "phase_ms_per_batch_sweep": phases,
+           "launch": "direct" if args.direct else "graph replay"}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_c1(d, A, args.seed)
     return res
@@ -317,6 +330,10 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--direct", action="store_true",
+                    help="c5 / c1batch: launch the timed sweeps one kernel at a time instead of "
+                         "replaying HIP graphs, and skip the one-chain-at-a-time runs (for "
+                         "kernel tracers)")
     ap.add_argument("--batch-sweeps", type=int, default=10)
     ap.add_argument("--tune", action="store_true",
                     help="search the tile shapes (structure_amd.tune) before timing")

@@ -1124,8 +1124,10 @@ __global__ void k_reduce_counts(const uint16_t *__restrict__ part, int nparts, s
     out[j] = s;
 }
 
-// Allele counts per locus for InitFreqPriors (Init.c:197-227), one thread per
-// locus; eps = lambda without the F model.
+// Allele counts per locus for InitFreqPriors (Init.c:197-227), one wavefront
+// per locus (four per block), lanes striding over the word rows; eps = lambda
+// without the F model.  The counts are exact integers: per-lane registers for
+// 2-bit words (popcounts of the code bits), LDS bins for byte words.
 __global__ __launch_bounds__(kNT) void k_init_eps(const ChainDesc *__restrict__ cds,
                                                    const BlockRef *__restrict__ map,
                                                    const double *__restrict__ lambda0s)
@@ -1135,56 +1137,71 @@ __global__ __launch_bounds__(kNT) void k_init_eps(const ChainDesc *__restrict__ 
     const SGPU_G int32_t *nall = g(C.nall);
     SGPU_G double *eps = g(C.eps);
     const double lambda0 = lambda0s[br_.chain];
-    const int Lpad = C.Lpad;
-    const int l = bx * kNT + threadIdx.x;
-    if (l >= m.L) {
+    __shared__ uint32_t s_bins[kNT / 64][kMaxAlleles];
+    const int wid = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int l = bx * (kNT / 64) + wid;
+    const bool live = l < m.L;   // wave-uniform
+    const int na = live ? nall[l] : 0;
+    uint32_t *bins = s_bins[wid];
+    for (int aa = lane; aa < m.A; aa += 64) {
+        bins[aa] = 0u;
+    }
+    __syncthreads();
+    double total = 0.0, c0 = 0.0, c1 = 0.0;
+    if (live && m.freqscorr) {
+        if (C.fmt == 1) {   // words [G][Lpad]: 16 individuals x 2 bits, code 3 = missing
+            const int G = (m.N + 15) / 16;
+            uint32_t nvalid = 0, ncode = 0;   // valid individuals, sum of their codes
+            for (int gr = lane; gr < G; gr += 64) {
+                const uint32_t word = geno[(size_t)gr * C.Lpad + l];
+                const int nreal = min(16, m.N - gr * 16);
+                const uint32_t livebits =
+                    nreal >= 16 ? 0x55555555u : (((1u << (2 * nreal)) - 1u) & 0x55555555u);
+                const uint32_t lo = word & 0x55555555u, hi = (word >> 1) & 0x55555555u;
+                const uint32_t valid = ~(lo & hi) & livebits;
+                nvalid += (uint32_t)__popc(valid);
+                ncode += (uint32_t)__popc(lo & valid) + 2u * (uint32_t)__popc(hi & valid);
+            }
+            // wave_sum leaves the sum in lane 0: broadcast it to the lanes that write
+            const double v = __shfl(wave_sum((double)nvalid), 0, 64);
+            const double cs = __shfl(wave_sum((double)ncode), 0, 64);
+            total = 2.0 * v;
+            c0 = 2.0 * v - cs;   // code c holds 2 - c copies of allele 0 and c of allele 1
+            c1 = cs;
+        } else {            // words [L][Gpad]: 4 copies of individuals 2h, 2h+1
+            const int G = (m.N + 1) / 2;
+            uint32_t n = 0;
+            for (int h = lane; h < G; h += 64) {
+                const uint32_t word = geno[(size_t)l * C.Gpad + h];
+                for (int c = 0; c < 4 && h * 2 + (c >> 1) < m.N; c++) {
+                    const int x = (int)((word >> (8 * c)) & 0xffu);
+                    if (x != 0xff) {
+                        n++;
+                        if (x < na) {
+                            atomicAdd(&bins[x], 1u);
+                        }
+                    }
+                }
+            }
+            total = __shfl(wave_sum((double)n), 0, 64);
+        }
+    }
+    __syncthreads();
+    if (!live) {
         return;
     }
-    const int na = nall[l];
     SGPU_G double *e = eps + (size_t)l * m.A;
-    for (int aa = 0; aa < m.A; aa++) {
-        e[aa] = 0.0;
-    }
-    if (!m.freqscorr) {
-        for (int aa = 0; aa < na; aa++) {
-            e[aa] = lambda0;
-        }
-        return;
-    }
-    int total = 0;
-    // counts accumulate into e[] as integers first (exact in double)
-    if (C.fmt == 1) {
-        const int G = (m.N + 15) / 16;
-        for (int g = 0; g < G; g++) {
-            const uint32_t word = geno[(size_t)g * Lpad + l];
-            for (int j = 0; j < 16 && g * 16 + j < m.N; j++) {
-                const uint32_t code = (word >> (2 * j)) & 3u;
-                if (code != 3u) {
-                    total += 2;
-                    e[0] += (double)(2 - (int)code);
-                    if (na > 1) {
-                        e[1] += (double)code;
-                    }
-                }
+    for (int aa = lane; aa < m.A; aa += 64) {
+        double v = 0.0;
+        if (aa < na) {
+            if (!m.freqscorr) {
+                v = lambda0;
+            } else {
+                const double cnt = C.fmt == 1 ? (aa == 0 ? c0 : c1) : (double)bins[aa];
+                v = (lambda0 + cnt) / ((double)na * lambda0 + total);
             }
         }
-    } else {
-        const int G = (m.N + 1) / 2;
-        for (int g = 0; g < G; g++) {
-            const uint32_t word = geno[(size_t)l * C.Gpad + g];   // byte words are [L][Gpad]
-            for (int c = 0; c < 4 && g * 2 + (c >> 1) < m.N; c++) {
-                const int x = (int)((word >> (8 * c)) & 0xffu);
-                if (x != 0xff) {
-                    total++;
-                    if (x < na) {
-                        e[x] += 1.0;
-                    }
-                }
-            }
-        }
-    }
-    for (int aa = 0; aa < na; aa++) {
-        e[aa] = ((lambda0 + e[aa]) / ((double)na * lambda0 + (double)total));
+        e[aa] = v;
     }
 }
 

@@ -869,7 +869,9 @@ int build_plans(sgpu_ctx *c)
                          })) ||
         (rc = build_plan(c, c->p_fst_fin, [](const HostChain &h) { return h.m.freqscorr != 0; }, one)) ||
         (rc = build_plan(c, c->p_collect, all_chains, one)) ||
-        (rc = build_plan(c, c->p_init_eps, all_chains, lb))) {
+        (rc = build_plan(c, c->p_init_eps, all_chains, [](const HostChain &h) {
+             return std::make_pair((int)cdiv(h.m.L, kNT / 64), 1);   // a wavefront per locus
+         }))) {
         return rc;
     }
     c->p_logq_chain.resize(c->chains.size());
