@@ -50,3 +50,33 @@ def test_count_invariants_at_full_size(data):
     assert np.array_equal(nQ.sum(axis=1), per_ind)
     # allele-1 copies per locus are fixed by the data whatever the ancestry
     assert np.array_equal(nA[:, :, 1].sum(axis=1), (g == 1).sum(axis=(0, 1)))
+
+
+@pytest.mark.parametrize("fmt,N2,L2,K2", [("snp", 150000, 3000, 4), ("byte", 60000, 700, 5)])
+def test_count_invariants_at_large_n(fmt, N2, L2, K2):
+    """Many individuals (up to 150,000): every index of the tile partials and of
+    the individual-major arrays stays in range, the sweep's counts still cover
+    every non-missing copy exactly once, and lnP(D) matches the oracle."""
+    print(f"large-N case {fmt} N={N2} L={L2} K={K2}", flush=True)
+    if fmt == "snp":
+        d = sa.synth_snp(N2, L2, Ktrue=K2, qalpha=0.5, miss=0.02, seed=9, want_geno=True)
+        g, nall, A, words, f = d["geno"], d["nall"], 2, d["words"], sa.GENO_SNP2
+    else:
+        d = sa.synth_microsat(N2, L2, amin=2, amax=12, Ktrue=K2, miss=0.02, seed=9)
+        g, nall, A = d["geno"], d["nall"], d["A"]
+        f, words = sa.pack(g, sa.GENO_BYTE)
+    m = sa.make_model(N2, L2, K2, A, freqscorr=1, inferalpha=1, metrofreq=2, burnin=2, seed=7)
+    ch = sa.Chain(m, f, words, nall)
+    ch.init()
+    ch.run(0, 5)   # Metropolis and Gibbs Q updates, three collecting sweeps
+    nA, nQ = ch.get(sa.NA), ch.get(sa.NQ)
+    assert np.array_equal(nA.sum(axis=(1, 2)), (g != -9).sum(axis=(0, 1)))
+    assert np.array_equal(nQ.sum(axis=1), (g != -9).sum(axis=(1, 2)))
+    for a in range(A):
+        assert np.array_equal(nA[:, :, a].sum(axis=1), (g == a).sum(axis=(0, 1)))
+    Q, P = ch.get(sa.Q), ch.get(sa.P)
+    och = oracle.Chain(oracle.make_model(N2, L2, K2, A), g, nall)
+    ref = och.calc_like(Q, P)
+    like = ch.get(sa.LIKE)[0]
+    assert abs(like - ref) <= 1e-9 * abs(ref)
+    ch.close()
