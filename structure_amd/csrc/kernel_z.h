@@ -97,13 +97,8 @@ struct LikeAcc {
 // Q[i,:] (wave uniform) is read through the constant address space, so it is
 // fetched with scalar loads.
 // ---------------------------------------------------------------------------
-// At K = 5 the kernel needs 87 VGPRs, five wavefronts per SIMD; held to 80 (six
-// wavefronts, a few spills outside the inner loop) it ran 1.4% faster at N=2000,
-// L=100000.  Other K are left alone (K <= 4 already fits 80; K >= 7 would spill
-// in the inner loop).
 template <int K, bool INIT, bool LIKE, bool ZOUT>
-__global__ __launch_bounds__(kZLoci)
-__attribute__((amdgpu_waves_per_eu(K == 5 ? 6 : 1, K == 5 ? 6 : 8))) void k_update_z_snp2(const ChainDesc *__restrict__ cds,
+__global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__restrict__ cds,
                                                           const BlockRef *__restrict__ map,
                                                           const uint32_t *__restrict__ repp)
 {
