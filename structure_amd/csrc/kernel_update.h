@@ -541,10 +541,14 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__res
         if (!active) {
             continue;
         }
+        // words [L][Gpad]: lanes 2j and 2j+1 read the same word, adjacent lanes
+        // adjacent words; the next locus's word is loaded one iteration ahead
+        uint32_t next = geno[(size_t)c0 * Gpad + (i >> 1)];
         for (int l = c0; l < c0 + nc; l++) {
-            // words [L][Gpad]: lanes 2j and 2j+1 read the same word, adjacent lanes
-            // adjacent words
-            const uint32_t word = geno[(size_t)l * Gpad + (i >> 1)];
+            const uint32_t word = next;
+            if (l + 1 < c0 + nc) {
+                next = geno[(size_t)(l + 1) * Gpad + (i >> 1)];
+            }
             const int xa = (int)((word >> sh) & 0xffu);
             const int xb = (int)((word >> (sh + 8)) & 0xffu);
             const double *pl = s_p + (size_t)(l - c0) * A * K;
