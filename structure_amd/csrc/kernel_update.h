@@ -451,8 +451,13 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__res
             s_p[t] = P[(size_t)c0 * K * 2 + t];
         }
         __syncthreads();
+        // four loci per load; the next four are loaded one iteration ahead
+        uint4 nw4 = *(const SGPU_G uint4 *)(row + c0);
         for (int lb = c0; lb < c1; lb += 4) {
-            const uint4 w4 = *(const SGPU_G uint4 *)(row + lb);
+            const uint4 w4 = nw4;
+            if (lb + 4 < c1) {
+                nw4 = *(const SGPU_G uint4 *)(row + lb + 4);
+            }
             const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
             for (int u = 0; u < 4; u++) {
