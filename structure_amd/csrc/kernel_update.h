@@ -954,10 +954,18 @@ __global__ __launch_bounds__(kNT) void k_fst_partial(const ChainDesc *__restrict
         const double nfr = (1.0 - newf) / newf;
         const double ofr = (1.0 - oldf) / oldf;
         const int p2end = m.onefst ? m.K : p1 + 1;
+        const SGPU_G double *el = eps + (size_t)l * m.A;
         for (int p2 = p1; p2 < p2end; p2++) {
+            const SGPU_G double *pl = P + ((size_t)l * m.K + p2) * m.A;
+            // the next allele's two values are loaded one iteration ahead
+            double ep_n = el[0], p_n = pl[0];
             for (int aa = 0; aa < na; aa++) {
-                const double ep = eps[(size_t)l * m.A + aa];
-                const double lp = log_d(P[((size_t)l * m.K + p2) * m.A + aa]);
+                const double ep = ep_n, pv = p_n;
+                if (aa + 1 < na) {
+                    ep_n = el[aa + 1];
+                    p_n = pl[aa + 1];
+                }
+                const double lp = log_d(pv);
                 v += (nfr * ep * lp - lgamma_lanczos(nfr * ep)) -
                      (ofr * ep * lp - lgamma_lanczos(ofr * ep));
             }
