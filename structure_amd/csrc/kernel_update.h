@@ -288,14 +288,29 @@ __global__ __launch_bounds__(kNT) void k_update_q_gibbs(const ChainDesc *__restr
     double x = 0.0;
     if (active) {
         const size_t n = (size_t)m.N * K;
-        const SGPU_G int32_t *f = g(C.nQfold) + (size_t)i * K + k;
         int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+        if (C.q_direct) {   // few locus tiles: their partials, four loads in flight
+            const SGPU_G uint16_t *f = g(C.nQpart) + (size_t)i * K + k;
+            const int nrows = C.n_lt;
+            int t = 0;
+            for (; t + 4 <= nrows; t += 4) {
+                c0 += f[(size_t)t * n];
+                c1 += f[(size_t)(t + 1) * n];
+                c2 += f[(size_t)(t + 2) * n];
+                c3 += f[(size_t)(t + 3) * n];
+            }
+            for (; t < nrows; t++) {
+                c0 += f[(size_t)t * n];
+            }
+        } else {            // the kRowSplit rows of the fold
+            const SGPU_G int32_t *f = g(C.nQfold) + (size_t)i * K + k;
 #pragma unroll
-        for (int s2 = 0; s2 < kRowSplit; s2 += 4) {
-            c0 += f[(size_t)s2 * n];
-            c1 += f[(size_t)(s2 + 1) * n];
-            c2 += f[(size_t)(s2 + 2) * n];
-            c3 += f[(size_t)(s2 + 3) * n];
+            for (int s2 = 0; s2 < kRowSplit; s2 += 4) {
+                c0 += f[(size_t)s2 * n];
+                c1 += f[(size_t)(s2 + 1) * n];
+                c2 += f[(size_t)(s2 + 2) * n];
+                c3 += f[(size_t)(s2 + 3) * n];
+            }
         }
         x = gamma_draw(alpha[k] + (double)((c0 + c1) + (c2 + c3)), m.seed, m.chain, rep,
                        (uint32_t)i, (uint32_t)k, PH_Q);

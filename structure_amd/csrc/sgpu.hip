@@ -63,6 +63,8 @@ constexpr int kZBlocksDefault = 4096;
 constexpr int kMetroBlocksDefault = 1024;
 // LDS per UpdateZ block for the per-individual counts (keeps 3 blocks per CU).
 constexpr size_t kZLdsMax = 48 * 1024;
+// Most UpdateZ locus tiles whose ancestry partials the Gibbs update sums itself.
+constexpr int kQDirectRows = 32;
 // Most loci one Metropolis-ratio block walks.
 constexpr long long kMetroMaxLoci = 2048;
 // LDS per byte-format UpdateZ block for the staged frequencies and counts, and
@@ -797,6 +799,9 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         d.z_chunk = (int)chunk;
         h.z_lds = (size_t)d.z_chunk * per_locus;
     }
+    // the Gibbs update sums up to kQDirectRows ancestry partials itself; more are
+    // first folded to kRowSplit rows by their own pass
+    d.q_direct = d.n_lt <= kQDirectRows ? 1 : 0;
     d.q_blocks = (int)cdiv(N, kNT);
     d.l_blocks = (int)cdiv(L, kNT);
     {
@@ -850,7 +855,7 @@ int build_plans(sgpu_ctx *c)
     auto lb = [](const HostChain &h) { return std::make_pair(h.d.l_blocks, 1); };
     if ((rc = build_plan(c, c->p_update_p, all_chains,
                          [](const HostChain &h) { return std::make_pair(h.d.p_blocks, 1); })) ||
-        (rc = build_plan(c, c->p_fold, all_chains,
+        (rc = build_plan(c, c->p_fold, [](const HostChain &h) { return h.d.q_direct == 0; },
                          [](const HostChain &h) {
                              return std::make_pair((int)cdiv((size_t)h.m.N * h.m.K, kNT), kRowSplit);
                          })) ||

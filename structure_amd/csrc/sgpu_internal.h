@@ -47,6 +47,7 @@ struct ChainDesc {
     int Lpad, G;                   // padded loci per word row (2-bit), word rows
     int Gpad;                      // byte format: padded word rows, the stride between loci
     int n_lt, n_it, rows_per_tile; // UpdateZ tiles: loci tiles, individual tiles, rows per tile
+    int q_direct;                  // Gibbs Q sums the n_lt ancestry partials itself (no fold)
     int zt_loci, z_chunk;          // loci per UpdateZ block; byte path: loci staged in LDS at once
     int q_blocks, l_blocks;        // cdiv(N, 256), cdiv(L, 256)
     int n_ranges, loci_per_range;  // Metropolis locus ranges
