@@ -112,6 +112,9 @@ def _declare(L: C.CDLL) -> None:
         "orc_calc_like": (C.c_double, [_M, _B, _D, _D]),
         "orc_collect": (None, [_M, _B, _I, C.POINTER(State)]),
         "orc_sweep": (None, [_M, _B, _I, u32, C.POINTER(State)]),
+        "orc_calc_like_snp2": (C.c_double, [_M, _U, C.c_int, _D, _D, C.c_int]),
+        "orc_snp2_counts": (None, [C.c_int, C.c_int, C.c_int, _U, C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -165,6 +168,37 @@ def gamma(a, seed, chain, c0, c1, c2, phase) -> float:
 
 def normal(seed, chain, c0, c1, c2, phase) -> float:
     return lib().orc_normal(seed, chain, c0, c1, c2, phase)
+
+
+# ---- 2-bit genotype words (large data, host threads) ---------------------------
+
+def _lpad(words: np.ndarray, N: int) -> int:
+    rows = (N + 15) // 16
+    assert words.size % rows == 0, "words must be [ceil(N/16)][Lpad]"
+    return words.size // rows
+
+
+def calc_like_snp2(model: Model, words: np.ndarray, Q, P, threads=0) -> float:
+    """CalcLike (CalcLike.c:387-426) of (Q, P) on 2-bit words [ceil(N/16)][Lpad],
+    equal bit for bit to Chain.calc_like on the unpacked genotypes."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    assert Q.shape == (model.N, model.K) and P.shape == (model.L, model.K, model.A)
+    return lib().orc_calc_like_snp2(C.byref(model), ptr(w, C.c_uint32), _lpad(w, model.N),
+                                    dp(Q), dp(P), int(threads))
+
+
+def snp2_counts(words: np.ndarray, N: int, L: int, threads=0):
+    """(non-missing copies per locus, allele-1 copies per locus, non-missing
+    copies per individual) of 2-bit words [ceil(N/16)][Lpad], as int64."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    lv = np.zeros(L, np.int64)
+    la = np.zeros(L, np.int64)
+    iv = np.zeros(N, np.int64)
+    lib().orc_snp2_counts(N, L, _lpad(w, N), ptr(w, C.c_uint32), ptr(lv, C.c_int64),
+                          ptr(la, C.c_int64), ptr(iv, C.c_int64), int(threads))
+    return lv, la, iv
 
 
 # ---- chain ------------------------------------------------------------------

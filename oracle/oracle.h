@@ -134,6 +134,17 @@ void orc_collect(const orc_model *m, const int8_t *geno, const int32_t *nall, or
 void orc_sweep(const orc_model *m, const int8_t *geno, const int32_t *nall, uint32_t rep,
                orc_state *s);
 
+/* --- 2-bit genotype words, host threads (oracle_packed.c) -----------------
+ * words [ceil(N/16)][Lpad]: individual 16g+t at bits 2t..2t+1 of word (g, l),
+ * code 0 = (0,0), 1 = (0,1), 2 = (1,1), 3 = missing.  nthreads <= 0: one per
+ * online CPU (at most 64). */
+double orc_calc_like_snp2(const orc_model *m, const uint32_t *words, int Lpad, const double *Q,
+                          const double *P, int nthreads);
+/* Non-missing copies per locus, allele-1 copies per locus, non-missing copies
+ * per individual. */
+void orc_snp2_counts(int N, int L, int Lpad, const uint32_t *words, int64_t *loc_valid,
+                     int64_t *loc_a1, int64_t *ind_valid, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1102,16 +1102,23 @@ int sgpu_destroy(sgpu_ctx *c)
     if (c->stream) {
         (void)hipStreamSynchronize(c->stream);
     }
+    // the executable graphs' kernel nodes hold addresses of the buffers below:
+    // release the graphs first, then the memory they refer to (the reference
+    // likewise releases its kernels before its buffers, Kernels.c:177-193)
+    for (int t = 0; t < 4; t++) {
+        if (c->graph[t]) {
+            (void)hipGraphExecDestroy(c->graph[t]);
+            c->graph[t] = nullptr;
+        }
+    }
+    if (c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+    }
     for (void *p : c->allocs) {
         (void)hipFree(p);
     }
     if (c->zdbg) {
         (void)hipFree(c->zdbg);
-    }
-    for (int t = 0; t < 4; t++) {
-        if (c->graph[t]) {
-            (void)hipGraphExecDestroy(c->graph[t]);
-        }
     }
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.a);

@@ -50,3 +50,32 @@ def z_uniforms(seed, chain, rep, N, L):
 def geno_ref_layout(geno):
     """[N][2][L] int8 -> int32 in the reference GenPos layout (structure.h:22)."""
     return np.ascontiguousarray(geno.astype(np.int32))
+
+
+def random_snp_codes(rng, N, L, miss=0.05):
+    """2-bit genotype codes [N][L] (0 = (0,0), 1 = (0,1), 2 = (1,1), 3 = missing)
+    and the same genotypes as [N][2][L] int8 (-9 missing)."""
+    codes = rng.integers(0, 3, size=(N, L)).astype(np.uint32)
+    codes[rng.random((N, L)) < miss] = 3
+    geno = np.empty((N, 2, L), np.int8)
+    geno[:, 0, :] = (codes == 2)
+    geno[:, 1, :] = (codes != 0)
+    miss_i, miss_l = np.nonzero(codes == 3)
+    geno[miss_i, :, miss_l] = -9
+    return codes, geno
+
+
+def pack_snp2_words(codes, Lpad=None):
+    """Words [ceil(N/16)][Lpad] of 2-bit codes [N][L]: individual 16g+t at bits
+    2t..2t+1 of word (g, l); padding individuals and loci read 3 (missing)."""
+    N, L = codes.shape
+    if Lpad is None:
+        Lpad = (L + 255) // 256 * 256
+    G = (N + 15) // 16
+    full = np.full((G * 16, Lpad), 3, np.uint32)
+    full[:N, :L] = codes
+    full = full.reshape(G, 16, Lpad)
+    words = np.zeros((G, Lpad), np.uint32)
+    for t in range(16):
+        words |= full[:, t, :] << np.uint32(2 * t)
+    return words
