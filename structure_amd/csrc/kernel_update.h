@@ -107,8 +107,8 @@ __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ 
     if (C.pslots) {   // work list: a group's nall[l] threads are adjacent, allele order
         const uint32_t e = g(C.pslots)[(size_t)bx * kNT + threadIdx.x];
         live = e != 0xFFFFFFFFu;
-        l = live ? (int)(e >> 12) : 0;
-        k = live ? (int)((e >> 8) & 15u) : 0;
+        l = live ? (int)(e >> 13) : 0;
+        k = live ? (int)((e >> 8) & 31u) : 0;
         aa = (int)(e & 255u);
     } else {          // dense grid: A threads per (locus, pop) group
         const int gpb = kNT / A;   // groups per block
@@ -425,7 +425,10 @@ __global__ __launch_bounds__(kNT) void k_metro_propose(const ChainDesc *__restri
     }
 }
 
-constexpr int kMetroLoci = 256;   // loci of P staged in LDS at a time
+// Loci of P staged in LDS at a time by the 2-bit Metropolis ratio: 256 up to
+// K = 16 (64 KiB, the most one block may hold), 128 above.
+template <int K>
+constexpr int metro_loci() { return K <= 16 ? 256 : 128; }
 
 // Biallelic 2-bit path, K templated: the block's P columns are staged in LDS
 // (wave-uniform reads), four loci per genotype load, fused multiply-adds in
@@ -452,6 +455,7 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__res
         tq[k] = active ? qprop[(size_t)i * K + k] : 1.0;
         cq[k] = active ? Q[(size_t)i * K + k] : 1.0;
     }
+    constexpr int kMetroLoci = metro_loci<K>();
     __shared__ double s_p[kMetroLoci * K * 2];
     const int l0 = by * loci_per_range;
     const int l1 = min(l0 + loci_per_range, m.L);
@@ -514,8 +518,10 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__res
 // chunk of loci are staged in LDS allele-major ([locus][allele][pop]): a copy
 // reads its allele's K values at constant offsets, and the block reads each
 // frequency from global memory once instead of once per lane.
-constexpr int kMetroByteLds = 4096;   // doubles of staged frequencies (32 KiB)
-static_assert(kMetroByteLds >= kMaxAlleles * kMaxK, "one locus must fit the staging buffer");
+// Doubles of staged frequencies: 32 KiB, or one locus of MAXALLELES x K where
+// that is more (K > 16 at 255 alleles: up to 40 KiB).
+template <int K>
+constexpr int metro_byte_lds() { return kMaxAlleles * K > 4096 ? kMaxAlleles * K : 4096; }
 
 template <int K>
 __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__restrict__ cds,
@@ -538,6 +544,7 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__res
         tq[k] = active ? qprop[(size_t)i * K + k] : 1.0;
         cq[k] = active ? Q[(size_t)i * K + k] : 1.0;
     }
+    constexpr int kMetroByteLds = metro_byte_lds<K>();
     __shared__ double s_p[kMetroByteLds];
     const int chunk = kMetroByteLds / (A * K);   // >= 1 (A <= kMaxAlleles)
     const int l0 = by * loci_per_range;

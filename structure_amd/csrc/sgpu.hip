@@ -284,6 +284,10 @@ zkern_t z_kernel(int fmt, int K, int mode)
     case 14: return pick_z<14>(fmt, mode);
     case 15: return pick_z<15>(fmt, mode);
     case 16: return pick_z<16>(fmt, mode);
+    case 17: return pick_z<17>(fmt, mode);
+    case 18: return pick_z<18>(fmt, mode);
+    case 19: return pick_z<19>(fmt, mode);
+    case 20: return pick_z<20>(fmt, mode);
     default: return nullptr;
     }
 }
@@ -315,6 +319,10 @@ mkern_t metro_kernel(int fmt, int K)
     case 14: return pick_metro<14>(fmt);
     case 15: return pick_metro<15>(fmt);
     case 16: return pick_metro<16>(fmt);
+    case 17: return pick_metro<17>(fmt);
+    case 18: return pick_metro<18>(fmt);
+    case 19: return pick_metro<19>(fmt);
+    case 20: return pick_metro<20>(fmt);
     default: return nullptr;
     }
 }
@@ -714,7 +722,7 @@ int build_p_slots(sgpu_ctx *c, HostChain &h, const Dataset &ds)
     const int gpb = kNT / A;
     d.p_blocks = (int)cdiv((size_t)L * K, (size_t)gpb);
     d.pslots = nullptr;
-    if (L >= (1 << 20) || A > 255) {
+    if (L >= (1 << 19) || A > 255 || K > 31) {
         return SGPU_OK;   // the fields of a list entry
     }
     std::vector<uint32_t> v;
@@ -727,7 +735,7 @@ int build_p_slots(sgpu_ctx *c, HostChain &h, const Dataset &ds)
                 fill = 0;
             }
             for (int a = 0; a < na; a++) {
-                v.push_back((uint32_t)l << 12 | (uint32_t)k << 8 | (uint32_t)a);
+                v.push_back((uint32_t)l << 13 | (uint32_t)k << 8 | (uint32_t)a);
             }
             fill += na;
         }
@@ -948,7 +956,7 @@ const char *sgpu_last_error(void) { return g_err; }
 
 const char *sgpu_version(void)
 {
-    return "sgpu 0.2 gfx950: chain-batched launches, fused UpdateZ (SNP2 and byte formats, K<=16), "
+    return "sgpu 0.3 gfx950: chain-batched launches, fused UpdateZ (SNP2 and byte formats, K<=20), "
            "FP64 draws";
 }
 

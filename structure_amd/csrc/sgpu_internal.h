@@ -26,7 +26,7 @@ constexpr double kLambdaPropSd = 0.3;    // structure.h:18
 constexpr double kLn2 = 0.6931471805599453;
 
 constexpr int kZLoci = 256;        // loci per UpdateZ block (one locus per lane)
-constexpr int kMaxK = 16;          // largest K the kernels take
+constexpr int kMaxK = 20;          // largest K the kernels take (MAXPOPS; UpdateZ.c:41-46)
 constexpr int kMaxAlleles = 255;   // byte format
 
 struct ModelArgs {
@@ -53,7 +53,7 @@ struct ChainDesc {
     int n_ranges, loci_per_range;  // Metropolis locus ranges
     int nhist, nloci_nonzero;      // allele-count histogram size, loci with alleles
     int p_blocks;                  // UpdateP blocks of this chain
-    // UpdateP work list [p_blocks][256]: (l << 12 | k << 8 | a) for a < nall[l],
+    // UpdateP work list [p_blocks][256]: (l << 13 | k << 8 | a) for a < nall[l],
     // each (locus, pop) group whole within one block, ~0 = idle thread; null:
     // the dense (locus, pop, allele < A) grid
     const uint32_t *pslots;

@@ -49,7 +49,9 @@ def _state(gch, nall, rng, K, A, fq=1.0):
 
 
 CASES_Z = [("snp", 1), ("snp", 2), ("snp", 5), ("snp", 10), ("byte", 3), ("byte", 7),
-           ("byteraw", 4), ("byteraw", 12)]
+           ("byteraw", 4), ("byteraw", 12),
+           # MAXPOPS above 16, up to the 20 of the north star (UpdateZ.c:41-46)
+           ("snp", 17), ("snp", 20), ("byte", 17), ("byte", 20), ("byteraw", 19)]
 
 
 @pytest.mark.parametrize("kind,K", CASES_Z)
@@ -278,6 +280,8 @@ TINY = [  # (format, N, L, K): single individuals and loci, odd N, L off every t
     ("snp", 1, 1, 1), ("snp", 1, 5, 2), ("snp", 3, 1, 3), ("snp", 17, 3, 2),
     ("snp", 33, 257, 4), ("snp", 2, 300, 16),
     ("byte", 1, 1, 2), ("byte", 3, 7, 3), ("byte", 5, 129, 5), ("byte", 2, 33, 16),
+    # K above 16: the largest register tiles, 128-locus Metropolis staging (2-bit)
+    ("snp", 40, 300, 17), ("snp", 33, 257, 20), ("byte", 9, 40, 17), ("byte", 5, 129, 20),
 ]
 
 
@@ -317,7 +321,8 @@ def test_tiny_and_odd_shapes_follow_the_oracle(fmt, N, L, K):
     _follow_oracle(fmt, N, L, K)
 
 
-@pytest.mark.parametrize("N,L,K,A", [(9, 11, 16, 120), (40, 6, 3, 127), (130, 3, 8, 64)])
+@pytest.mark.parametrize("N,L,K,A", [(9, 11, 16, 120), (40, 6, 3, 127), (130, 3, 8, 64),
+                                     (9, 11, 20, 127)])
 def test_many_alleles_follow_the_oracle(N, L, K, A):
     """MAXALLELES up to 127 (the int8 input's largest allele index + 1): one locus
     per LDS chunk in UpdateZ, two in the Metropolis ratio, one or two (locus, pop)

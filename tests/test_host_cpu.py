@@ -130,7 +130,7 @@ def test_create_rejects_bad_shapes_before_touching_the_device():
         sa.Chain(sa.make_model(20, 30, 2, 2), sa.GENO_SNP2, d["words"][:-1], d["nall"])
     assert e.value.code == sa.ERR_ARG
     with pytest.raises(sa.SgpuError) as e:
-        sa.Chain(sa.make_model(20, 30, 17, 2), sa.GENO_SNP2, d["words"], d["nall"])
+        sa.Chain(sa.make_model(20, 30, 21, 2), sa.GENO_SNP2, d["words"], d["nall"])   # K > 20
     assert e.value.code == sa.ERR_UNSUPPORTED
     bad = d["nall"].copy()
     bad[bad == 2] = 1                            # allele 1 present but NumAlleles says 1
