@@ -58,6 +58,16 @@ typedef struct {
     double alphapropsd, alphamax, alphapriora, alphapriorb;
     double fpriormean, fpriorsd;
     uint32_t seed, chain;         /* Philox key */
+    /* ANCESTDIST, NUMBOXES: histogram of each Q[i,k] over the collected sweeps in
+     * NUMBOXES equal boxes of [0,1] (UpdateSums, output.c:691-700); read back as
+     * SGPU_ANCESTDIST.  NUMBOXES is ignored when ANCESTDIST is 0. */
+    int32_t ancestdist, numboxes;
+    /* PFROMPOPFLAGONLY: the allele counts behind the P update take only the
+     * individuals whose POPFLAG is 1 (GetNumFromPops, UpdateP.c:85-88); the
+     * ancestry counts behind Q still take every individual.  Needs
+     * sgpu_data.popflag. */
+    int32_t pfrompopflagonly;
+    int32_t reserved;             /* 0 */
 } sgpu_model;
 
 typedef struct {
@@ -65,6 +75,9 @@ typedef struct {
     const uint32_t *geno;         /* packed words (host memory) */
     size_t geno_words;            /* number of uint32 words */
     const int32_t *nall;          /* [L] NumAlleles after recoding (datain.c:824-904) */
+    /* [N] POPFLAG column of the data file (Individual[i].PopFlag, datain.c), or
+     * NULL when the file has none; read only by chains with PFROMPOPFLAGONLY. */
+    const int32_t *popflag;
 } sgpu_data;
 
 typedef struct sgpu_ctx sgpu_ctx;
@@ -86,7 +99,8 @@ enum {
     SGPU_ALPHASUM = 13,  /* double [K]               */
     SGPU_LAMBDASUM = 14, /* double [K]               */
     SGPU_LIKE = 15,      /* double [3]: like, sumlikes, sumsqlikes (output.c:86-99) */
-    SGPU_NSUM = 16       /* int64  [1]: number of collected sweeps */
+    SGPU_NSUM = 16,      /* int64  [1]: number of collected sweeps */
+    SGPU_ANCESTDIST = 17 /* int32  [N][K][NUMBOXES]  AncestDist (structure.h:35); ANCESTDIST=1 */
 };
 
 /* Flags for sgpu_update_z. */

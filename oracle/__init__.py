@@ -49,7 +49,9 @@ class Model(C.Structure):
         "popalphas", "unifprioralpha", "inferlambda", "popspecificlambda", "metrofreq",
         "computeprob")] + [(n, C.c_double) for n in (
             "alpha0", "lambda0", "alphapropsd", "alphamax", "alphapriora", "alphapriorb",
-            "fpriormean", "fpriorsd")] + [("seed", C.c_uint32), ("chain", C.c_uint32)]
+            "fpriormean", "fpriorsd")] + [("seed", C.c_uint32), ("chain", C.c_uint32)] + [
+        ("ancestdist", C.c_int), ("numboxes", C.c_int), ("pfrompopflagonly", C.c_int),
+        ("popflag", C.POINTER(C.c_int32))]
 
 
 def make_model(N, L, K, A, **kw) -> Model:
@@ -58,7 +60,7 @@ def make_model(N, L, K, A, **kw) -> Model:
              unifprioralpha=1, inferlambda=0, popspecificlambda=0, metrofreq=10,
              computeprob=1, alpha0=1.0, lambda0=1.0, alphapropsd=0.025, alphamax=10.0,
              alphapriora=1.0, alphapriorb=2.0, fpriormean=0.01, fpriorsd=0.05, seed=1,
-             chain=0)
+             chain=0, ancestdist=0, numboxes=1000, pfrompopflagonly=0)
     d.update(kw)
     return Model(N=N, L=L, K=K, A=A, **d)
 
@@ -75,7 +77,7 @@ class State(C.Structure):
         ("nA", _I), ("nQ", _I)] + [(n, _D) for n in (
             "QSum", "PSum", "FstSum", "epsSum", "alphaSum", "lambdaSum")] + [
         ("like", C.c_double), ("sumlikes", C.c_double), ("sumsqlikes", C.c_double),
-        ("nsum", C.c_int64)]
+        ("nsum", C.c_int64), ("ancestDist", _I)]
 
 
 def _declare(L: C.CDLL) -> None:
@@ -205,11 +207,13 @@ def snp2_counts(words: np.ndarray, N: int, L: int, threads=0):
 
 @dataclass
 class Chain:
-    """One oracle chain: model, genotypes ([N][2][L] int8, -9 missing), state."""
+    """One oracle chain: model, genotypes ([N][2][L] int8, -9 missing), state;
+    popflag ([N], the data file's POPFLAG column) for PFROMPOPFLAGONLY."""
 
     model: Model
     geno: np.ndarray
     nall: np.ndarray
+    popflag: np.ndarray | None = None
     arrays: dict = field(default_factory=dict)
 
     def __post_init__(self):
@@ -218,12 +222,18 @@ class Chain:
         self.geno = np.ascontiguousarray(self.geno, dtype=np.int8)
         assert self.geno.shape == (N, 2, L)
         self.nall = np.ascontiguousarray(self.nall, dtype=np.int32)
+        if self.popflag is not None:
+            self.popflag = np.ascontiguousarray(self.popflag, dtype=np.int32)
+            assert self.popflag.shape == (N,)
+            m.popflag = ip(self.popflag)
         z = np.zeros
         self.arrays = dict(
             Q=z((N, K)), P=z((L, K, A)), eps=z((L, A)), F=z(K), alpha=z(K), lambda_=z(K),
             nA=z((L, K, A), np.int32), nQ=z((N, K), np.int32), QSum=z((N, K)),
             PSum=z((L, K, A)), FstSum=z(K), epsSum=z((L, A)), alphaSum=z(K),
             lambdaSum=z(K))
+        if m.ancestdist:
+            self.arrays["ancestDist"] = z((N, K, m.numboxes), np.int32)
         st = State()
         for name, arr in self.arrays.items():
             setattr(st, name, dp(arr) if arr.dtype == np.float64 else ip(arr))

@@ -146,9 +146,16 @@ double orc_lgamma(double z)
     return log(s) + 0.9189385332046727 - (z + 6.5) + (z - 0.5) * log(z + 6.5);
 }
 
+/* GetNumFromPops's filter (UpdateP.c:85-88): with PFROMPOPFLAGONLY only the
+ * individuals whose PopFlag is 1 add to the allele counts. */
+static int orc_counts_alleles(const orc_model *m, int ind)
+{
+    return !m->pfrompopflagonly || (m->popflag && m->popflag[ind] == 1);
+}
+
 /* ------------------------------------------------------------------------
- * Counts from an explicit Z: GetNumFromPops (UpdateP.c:60-99, all
- * individuals, PFROMPOPFLAGONLY=0) and GetNumLociPop (UpdateQ.c:17-35).
+ * Counts from an explicit Z: GetNumFromPops (UpdateP.c:60-99, with its
+ * PFROMPOPFLAGONLY filter) and GetNumLociPop (UpdateQ.c:17-35).
  * geno is [N][2][L] int8 with ORC_MISSING for missing copies.
  * ---------------------------------------------------------------------- */
 void orc_count_from_z(const orc_model *m, const int8_t *geno, const int32_t *Z,
@@ -163,7 +170,7 @@ void orc_count_from_z(const orc_model *m, const int8_t *geno, const int32_t *Z,
                 size_t g = ((size_t)i * 2 + c) * L + l;
                 int x = geno[g];
                 int z = Z[g];
-                if (x != ORC_MISSING && z != ORC_UNASSIGNED) {
+                if (x != ORC_MISSING && z != ORC_UNASSIGNED && orc_counts_alleles(m, i)) {
                     nA[((size_t)l * K + z) * A + x]++;
                 }
                 if (z != ORC_UNASSIGNED) {
@@ -214,7 +221,9 @@ void orc_update_z(const orc_model *m, const int8_t *geno, const double *Q, const
                     if (z >= K) {
                         z = K - 1;   /* reachable only with a NaN sum */
                     }
-                    nA[((size_t)l * K + z) * A + x]++;
+                    if (orc_counts_alleles(m, i)) {
+                        nA[((size_t)l * K + z) * A + x]++;
+                    }
                     nQ[(size_t)i * K + z]++;
                 }
                 if (Z) {
@@ -244,7 +253,9 @@ void orc_init_z(const orc_model *m, const int8_t *geno, int32_t *Z, int32_t *nA,
                 int z = ORC_UNASSIGNED;
                 if (x != ORC_MISSING) {
                     z = (int)(orc_uz(w[2 * (i & 1) + taken++]) * (double)K);
-                    nA[((size_t)l * K + z) * A + x]++;
+                    if (orc_counts_alleles(m, i)) {
+                        nA[((size_t)l * K + z) * A + x]++;
+                    }
                     nQ[(size_t)i * K + z]++;
                 }
                 if (Z) {
@@ -712,7 +723,8 @@ double orc_calc_like(const orc_model *m, const int8_t *geno, const double *Q, co
 }
 
 /* DataCollection (output.c:50-101) -> UpdateSums (output.c:654-707):
- * posterior sums plus the lnP(D) accumulators (ANCESTDIST=0, LOCPRIOR=0). */
+ * posterior sums, the ANCESTDIST histogram (output.c:691-700) and the lnP(D)
+ * accumulators (LOCPRIOR=0). */
 void orc_collect(const orc_model *m, const int8_t *geno, const int32_t *nall, orc_state *s)
 {
     const int N = m->N, L = m->L, K = m->K, A = m->A;
@@ -737,6 +749,18 @@ void orc_collect(const orc_model *m, const int8_t *geno, const int32_t *nall, or
         for (int l = 0; l < L; l++) {
             for (int a = 0; a < nall[l]; a++) {
                 s->epsSum[(size_t)l * A + a] += s->eps[(size_t)l * A + a];
+            }
+        }
+    }
+    if (m->ancestdist && s->ancestDist) {
+        const int nb = m->numboxes;
+        for (int i = 0; i < N; i++) {
+            for (int k = 0; k < K; k++) {
+                int box = (int)(s->Q[(size_t)i * K + k] * (double)nb);
+                if (box == nb) {
+                    box = nb - 1;   /* Q = 1.000 */
+                }
+                s->ancestDist[((size_t)i * K + k) * nb + box]++;
             }
         }
     }
@@ -810,6 +834,9 @@ void orc_init_state(const orc_model *m, const int8_t *geno, const int32_t *nall,
     memset(s->PSum, 0, sizeof(double) * (size_t)L * K * A);
     memset(s->epsSum, 0, sizeof(double) * (size_t)L * A);
     orc_init_z(m, geno, NULL, s->nA, s->nQ);
+    if (m->ancestdist && s->ancestDist) {   /* InitializeSums (Init.c:265-271) */
+        memset(s->ancestDist, 0, sizeof(int32_t) * (size_t)N * K * m->numboxes);
+    }
     s->like = s->sumlikes = s->sumsqlikes = 0.0;
     s->nsum = 0;
 }

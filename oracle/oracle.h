@@ -58,6 +58,9 @@ typedef struct {
     double alphapropsd, alphamax, alphapriora, alphapriorb;
     double fpriormean, fpriorsd;
     uint32_t seed, chain;
+    int ancestdist, numboxes;   /* ANCESTDIST histogram of Q (output.c:691-700) */
+    int pfrompopflagonly;       /* allele counts from POPFLAG=1 only (UpdateP.c:85-88) */
+    const int32_t *popflag;     /* [N] Individual[i].PopFlag, or NULL */
 } orc_model;
 
 /* Chain state.  Layouts follow structure.h:21-33 (GenPos, PPos, QPos, EpsPos). */
@@ -73,6 +76,7 @@ typedef struct {
     double *QSum, *PSum, *FstSum, *epsSum, *alphaSum, *lambdaSum; /* posterior sums */
     double like, sumlikes, sumsqlikes;
     int64_t nsum;     /* number of collected sweeps */
+    int32_t *ancestDist;   /* [N][K][NUMBOXES] (AncestDistPos, structure.h:35) when ANCESTDIST */
 } orc_state;
 
 /* --- RNG --------------------------------------------------------------- */

@@ -21,7 +21,7 @@ _LIB = None
 OK, ERR_ARG, ERR_NOMEM, ERR_DEVICE, ERR_STATE, ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 GENO_SNP2, GENO_BYTE = 1, 2
 (Q, P, EPS, FST, ALPHA, LAMBDA, NA, NQ, QSUM, PSUM, FSTSUM, EPSSUM, ALPHASUM, LAMBDASUM, LIKE,
- NSUM) = range(1, 17)
+ NSUM, ANCESTDIST) = range(1, 18)
 Z_LIKE = 1
 PHASES = ("P", "Q", "Z", "alpha", "lambda", "eps", "fst", "collect")
 
@@ -40,12 +40,16 @@ class Model(C.Structure):
         "popalphas", "unifprioralpha", "inferlambda", "popspecificlambda", "metrofreq",
         "computeprob")] + [(n, C.c_double) for n in (
             "alpha0", "lambda0", "alphapropsd", "alphamax", "alphapriora", "alphapriorb",
-            "fpriormean", "fpriorsd")] + [("seed", C.c_uint32), ("chain", C.c_uint32)]
+            "fpriormean", "fpriorsd")] + [("seed", C.c_uint32), ("chain", C.c_uint32)] + [
+        (n, C.c_int32) for n in ("ancestdist", "numboxes", "pfrompopflagonly", "reserved")]
 
 
 class Data(C.Structure):
+    """Mirror of ``sgpu_data``; popflag may be null."""
+
     _fields_ = [("format", C.c_int32), ("geno", C.POINTER(C.c_uint32)),
-                ("geno_words", C.c_size_t), ("nall", C.POINTER(C.c_int32))]
+                ("geno_words", C.c_size_t), ("nall", C.POINTER(C.c_int32)),
+                ("popflag", C.POINTER(C.c_int32))]
 
 
 class Tuning(C.Structure):
@@ -61,7 +65,7 @@ DEFAULTS = dict(burnin=0, numreps=0, freqscorr=1, onefst=0, inferalpha=1, popalp
                 unifprioralpha=1, inferlambda=0, popspecificlambda=0, metrofreq=10,
                 computeprob=1, alpha0=1.0, lambda0=1.0, alphapropsd=0.025, alphamax=10.0,
                 alphapriora=1.0, alphapriorb=2.0, fpriormean=0.01, fpriorsd=0.05, seed=1,
-                chain=0)
+                chain=0, ancestdist=0, numboxes=1000, pfrompopflagonly=0, reserved=0)
 
 
 def make_model(N, L, K, A, **kw) -> Model:
@@ -200,7 +204,7 @@ def _state_shape(m: Model, which):
             PSUM: ((L, K, A), np.float64), FSTSUM: ((K,), np.float64),
             EPSSUM: ((L, A), np.float64), ALPHASUM: ((K,), np.float64),
             LAMBDASUM: ((K,), np.float64), LIKE: ((3,), np.float64),
-            NSUM: ((1,), np.int64)}[which]
+            NSUM: ((1,), np.int64), ANCESTDIST: ((N, K, m.numboxes), np.int32)}[which]
 
 
 class _Context:
@@ -300,23 +304,29 @@ class Chain(_Context):
     UpdateEpsilonCL, ``update_fst`` ~ UpdateFstCL, ``collect`` ~ DataCollectionCL;
     ``get``/``set`` ~ readBuffer/writeBuffer."""
 
-    def __init__(self, model: Model, fmt: int, words: np.ndarray, nall: np.ndarray):
+    def __init__(self, model: Model, fmt: int, words: np.ndarray, nall: np.ndarray,
+                 popflag: np.ndarray | None = None):
         self.model = model
         self.models = [model]
         self._words = np.ascontiguousarray(words, dtype=np.uint32)
         self._nall = np.ascontiguousarray(nall, dtype=np.int32)
+        self._popflag = (None if popflag is None
+                         else np.ascontiguousarray(popflag, dtype=np.int32))
         data = Data(format=fmt, geno=self._words.ctypes.data_as(C.POINTER(C.c_uint32)),
                     geno_words=self._words.size,
-                    nall=self._nall.ctypes.data_as(C.POINTER(C.c_int32)))
+                    nall=self._nall.ctypes.data_as(C.POINTER(C.c_int32)),
+                    popflag=(None if self._popflag is None
+                             else self._popflag.ctypes.data_as(C.POINTER(C.c_int32))))
         h = C.c_void_p()
         _check(lib().sgpu_create(C.byref(model), C.byref(data), C.byref(h)))
         self._h = h
         self.fmt = fmt
 
     @classmethod
-    def from_geno(cls, model: Model, geno: np.ndarray, nall: np.ndarray, fmt=None):
+    def from_geno(cls, model: Model, geno: np.ndarray, nall: np.ndarray, fmt=None,
+                  popflag: np.ndarray | None = None):
         f, words = pack(geno, fmt)
-        return cls(model, f, words, nall)
+        return cls(model, f, words, nall, popflag)
 
     def get(self, which) -> np.ndarray:
         return self.get_chain(0, which)
@@ -329,8 +339,8 @@ class Batch(_Context):
     """Several chains advanced together (``sgpu_create_batch``): one launch per
     kernel and phase over all of them.
 
-    ``datasets`` is a list of ``(fmt, words, nall)``; chain c runs
-    ``models[c]`` on ``datasets[data_of_chain[c]]``."""
+    ``datasets`` is a list of ``(fmt, words, nall)`` or ``(fmt, words, nall,
+    popflag)``; chain c runs ``models[c]`` on ``datasets[data_of_chain[c]]``."""
 
     def __init__(self, models, datasets, data_of_chain=None, tuning=None):
         self.models = list(models)
@@ -339,12 +349,17 @@ class Batch(_Context):
             data_of_chain = [0] * n
         self._dsets = []
         darr = (Data * len(datasets))()
-        for j, (fmt, words, nall) in enumerate(datasets):
+        for j, ds in enumerate(datasets):
+            fmt, words, nall = ds[:3]
+            pf = ds[3] if len(ds) > 3 else None
             w = np.ascontiguousarray(words, dtype=np.uint32)
             na = np.ascontiguousarray(nall, dtype=np.int32)
-            self._dsets.append((w, na))
+            pfa = None if pf is None else np.ascontiguousarray(pf, dtype=np.int32)
+            self._dsets.append((w, na, pfa))
             darr[j] = Data(format=fmt, geno=w.ctypes.data_as(C.POINTER(C.c_uint32)),
-                           geno_words=w.size, nall=na.ctypes.data_as(C.POINTER(C.c_int32)))
+                           geno_words=w.size, nall=na.ctypes.data_as(C.POINTER(C.c_int32)),
+                           popflag=(None if pfa is None
+                                    else pfa.ctypes.data_as(C.POINTER(C.c_int32))))
         marr = (Model * n)(*self.models)
         self._doc = np.ascontiguousarray(data_of_chain, dtype=np.int32)
         assert self._doc.size == n

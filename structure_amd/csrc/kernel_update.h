@@ -171,6 +171,20 @@ __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ 
     }
 }
 
+// ANCESTDIST (UpdateSums, output.c:691-700): one more count in the box of
+// [0,1] that q falls into, NUMBOXES boxes, q = 1 in the last.  Each (i, k) has
+// one thread per launch, so the count is a plain read-modify-write.
+__device__ __forceinline__ void ancest_box(const ModelArgs &m, SGPU_G int32_t *adist, int i, int k,
+                                           double q)
+{
+    const int nb = m.numboxes;
+    int box = (int)(q * (double)nb);
+    if (box == nb) {
+        box = nb - 1;
+    }
+    adist[((size_t)i * m.K + k) * nb + box] += 1;
+}
+
 // ---------------------------------------------------------------------------
 // Dirichlet(alpha + counts) for one individual into q[] (UpdateQAdmixture,
 // UpdateQ.c:387-424 / RDirichlet ran.c:348-359), or the Metropolis proposal
@@ -199,7 +213,8 @@ __device__ __forceinline__ void dirichlet_q(const ModelArgs &m, PA alpha, const 
 template <int NT, class PQ>
 __device__ __forceinline__ void finish_q(const ModelArgs &m, int bx, int nrows, int i, bool active,
                                          const double *q, PQ Q, PQ QSum, int collect, PQ logq_part,
-                                         SGPU_G int32_t *logq_rows, double *s_red)
+                                         SGPU_G int32_t *logq_rows, SGPU_G int32_t *adist,
+                                         double *s_red)
 {
     double tot = 0.0;
     for (int k = 0; k < m.K; k++) {
@@ -207,6 +222,9 @@ __device__ __forceinline__ void finish_q(const ModelArgs &m, int bx, int nrows, 
             Q[(size_t)i * m.K + k] = q[k];
             if (collect) {
                 QSum[(size_t)i * m.K + k] += q[k];
+                if (m.ancestdist) {
+                    ancest_box(m, adist, i, k, q[k]);
+                }
             }
             tot += log_d(q[k] > kSqrtUnder ? q[k] : kSqrtUnder);
         }
@@ -263,8 +281,8 @@ __global__ __launch_bounds__(kNT) void k_fold_rows(const ChainDesc *__restrict__
 // ran.c:348-359): one thread per (individual, pop), a block holding whole
 // individuals (K * (256 / K) threads).  Each thread sums its folded ancestry
 // counts ([kRowSplit][N][K], loads in flight) and draws one gamma; the
-// individual's sum is taken in pop order from LDS.  Fused: QSum, and the
-// log-Q statistics of the alpha update, one row per block: per pop the sum
+// individual's sum is taken in pop order from LDS.  Fused: QSum, the ANCESTDIST
+// histogram, and the log-Q statistics of the alpha update, one row per block: per pop the sum
 // over the block's individuals of log Q[i][k] (LogProbQonepop,
 // UpdateAlpha.c:102-111), and the sum of the per-individual totals of
 // log max(Q, 1e-50) (LogProbQ, UpdateAlpha.c:157-180).
@@ -327,6 +345,9 @@ __global__ __launch_bounds__(kNT) void k_update_q_gibbs(const ChainDesc *__restr
         Q[(size_t)i * K + k] = q;
         if (collect) {
             QSum[(size_t)i * K + k] += q;
+            if (m.ancestdist) {
+                ancest_box(m, g(C.adist), i, k, q);
+            }
         }
     }
     s_l[threadIdx.x] = active ? log_d(q) : 0.0;
@@ -650,7 +671,7 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restric
         }
     }
     finish_q<kNT>(m, bx, C.q_blocks, i, active, q, Q, g(C.QSum), collect, g(C.logq_part),
-                  g(C.logq_rows), s_red);
+                  g(C.logq_rows), g(C.adist), s_red);
 }
 
 // ---------------------------------------------------------------------------

@@ -119,6 +119,9 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     SGPU_G uint16_t *nQpart = g(C.nQpart);
     SGPU_G double *likepart = g(C.likepart);
     SGPU_G int8_t *zout = g(C.zout);
+    // PFROMPOPFLAGONLY: per word row, bit 2t set when individual 16g+t adds to the
+    // allele counts (null: every individual does)
+    const SGPU_G uint32_t *pfmask = g(C.pfmask);
     extern __shared__ uint32_t s_cnt[];   // [tile individual][K]: valid copies, then z > k
     const int lane = threadIdx.x & 63;
     const int l = bx * kZLoci + threadIdx.x;   // < Lpad by construction
@@ -148,12 +151,14 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     for (int k = 0; k < K; k++) {
         TA[k] = 0;
     }
-    uint32_t Ttot = 0, A1tot = 0;
+    uint32_t Ttot = 0, A1tot = 0;   // copies (and allele-1 copies) the allele counts take
+    uint32_t Tall = 0;              // every valid copy (the lnP terms)
     LikeAcc acc;
 
     uint32_t word = geno[(size_t)row0 * Lpad + l];
     for (int g = row0; g < row1; g++) {
         const uint32_t next = (g + 1 < row1) ? geno[(size_t)(g + 1) * Lpad + l] : 0u;
+        const uint32_t pfm = pfmask ? pfmask[g] : 0x55555555u;   // wave-uniform
         {
             // this row's valid copies and allele-1 copies at the lane's locus, from the
             // word's bit pattern (code 3 = missing; only individuals < N count)
@@ -161,7 +166,9 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
             const uint32_t live =
                 nreal >= 16 ? 0x55555555u : (((1u << (2 * nreal)) - 1u) & 0x55555555u);
             const uint32_t lo = word & 0x55555555u, hi = (word >> 1) & 0x55555555u;
-            const uint32_t vmask = ~(lo & hi) & live;
+            const uint32_t vall = ~(lo & hi) & live;
+            const uint32_t vmask = vall & pfm;
+            Tall += 2u * (uint32_t)__popc(vall);
             Ttot += 2u * (uint32_t)__popc(vmask);
             A1tot += (uint32_t)__popc(lo & ~hi & vmask) + 2u * (uint32_t)__popc(hi & ~lo & vmask);
         }
@@ -189,6 +196,9 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                 const bool x0 = code == 2u;
                 const bool x1 = code != 0u;
                 const uint32_t X0 = x0 ? 0x10001u : 1u, X1 = x1 ? 0x10001u : 1u;
+                // lanes whose copies go into the allele counts: all, or none when the
+                // individual's POPFLAG is not 1 (its ancestry counts still take them)
+                const uint64_t fm = ((pfm >> (2 * t)) & 1u) ? ~0ull : 0ull;
                 const uint32_t wa = h ? w.z : w.x;
                 const uint32_t wb = h ? w.w : w.y;
                 const uint64_t vm = ballot(valid);
@@ -202,8 +212,9 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                     for (int k = 0; k < K - 1; k++) {
                         const bool m0 = valid && u0 > (uint32_t)k;
                         const bool m1 = valid && u1 > (uint32_t)k;
-                        TA[k] += (m0 ? X0 : 0u) + (m1 ? X1 : 0u);
-                        nk[k + 1] = (uint32_t)(__popcll(ballot(m0)) + __popcll(ballot(m1)));
+                        const uint64_t b0 = ballot(m0), b1 = ballot(m1);
+                        TA[k] += lane_select(b0 & fm, X0) + lane_select(b1 & fm, X1);
+                        nk[k + 1] = (uint32_t)(__popcll(b0) + __popcll(b1));
                     }
                 } else {
                     double pre0[K], pre1[K];
@@ -241,7 +252,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                         const uint64_t a1 = ballot(pre0[k] < r1), b1 = ballot(pre1[k] < r1);
                         const uint64_t m0 = (xm0 & b0) | (~xm0 & a0);   // z0 > k
                         const uint64_t m1 = (xm1 & b1) | (~xm1 & a1);   // z1 > k
-                        TA[k] += lane_select(m0, X0) + lane_select(m1, X1);
+                        TA[k] += lane_select(m0 & fm, X0) + lane_select(m1 & fm, X1);
                         nk[k + 1] = (uint32_t)(__popcll(m0) + __popcll(m1));
                         if (ZOUT) {
                             z0 += (int)((m0 >> lane) & 1u);
@@ -307,7 +318,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
         __shared__ double s_red[kZLoci / 64];
         // every valid copy's term was multiplied in scaled by 2^-32, every missing
         // individual as the clamp value squared
-        const double v = lok ? acc.log_value(32 * (int)Ttot, ntile - (int)(Ttot / 2u)) : 0.0;
+        const double v = lok ? acc.log_value(32 * (int)Tall, ntile - (int)(Tall / 2u)) : 0.0;
         const double tot = block_sum<kZLoci>(v, s_red);
         if (threadIdx.x == 0) {
             likepart[(size_t)by * n_lt + bx] = tot;
@@ -331,10 +342,13 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
 constexpr int kByteRows = 256;   // most word rows (individual pairs) per UpdateZ block; a batch
                                  // may use 128 or 64 (the block size)
 
+// One allele copy: its draw, its "z > k" counts (T, V) and, when `count`, its
+// bin of the locus's allele-by-population histogram (s_hl).
 template <int K, bool INIT, bool LIKE>
-__device__ __forceinline__ void byte_copy(int x, bool valid, uint32_t w, const double *q,
-                                          const double *s_pl, int A, uint32_t *s_hl,
-                                          uint32_t *T, uint32_t &V, LikeAcc &acc, int &zout)
+__device__ __forceinline__ void byte_copy(int x, bool valid, bool count, uint32_t w,
+                                          const double *q, const double *s_pl, int A,
+                                          uint32_t *s_hl, uint32_t *T, uint32_t &V, LikeAcc &acc,
+                                          int &zout)
 {
     if (!valid) {
         return;
@@ -373,7 +387,9 @@ __device__ __forceinline__ void byte_copy(int x, bool valid, uint32_t w, const d
             z += gt;
         }
     }
-    atomicAdd(&s_hl[z * A + x], 1u);
+    if (count) {
+        atomicAdd(&s_hl[z * A + x], 1u);
+    }
     zout = z;
 }
 
@@ -405,6 +421,10 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
     const bool rowok = h < G;
     const int i0 = 2 * h, i1 = 2 * h + 1;
     const bool ok0 = rowok && i0 < N, ok1 = rowok && i1 < N;
+    // PFROMPOPFLAGONLY: only individuals with POPFLAG 1 add to the allele counts
+    const SGPU_G int32_t *popflag = g(C.popflag);
+    const bool cnt0 = !popflag || (ok0 && popflag[i0] == 1);
+    const bool cnt1 = !popflag || (ok1 && popflag[i1] == 1);
     double q0[K], q1[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -456,12 +476,12 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
                 const bool va = ok0 && xa != 0xff, vb = ok0 && xb != 0xff;
                 const bool vc = ok1 && xc != 0xff, vd = ok1 && xd != 0xff;
                 int za = -9, zb = -9, zc = -9, zd = -9;
-                byte_copy<K, INIT, LIKE>(xa, va, w.x, q0, s_pl, A, s_hl, T0, V0, acc, za);
-                byte_copy<K, INIT, LIKE>(xb, vb, va ? w.y : w.x, q0, s_pl, A, s_hl, T0, V0, acc,
-                                         zb);
-                byte_copy<K, INIT, LIKE>(xc, vc, w.z, q1, s_pl, A, s_hl, T1, V1, acc, zc);
-                byte_copy<K, INIT, LIKE>(xd, vd, vc ? w.w : w.z, q1, s_pl, A, s_hl, T1, V1, acc,
-                                         zd);
+                byte_copy<K, INIT, LIKE>(xa, va, cnt0, w.x, q0, s_pl, A, s_hl, T0, V0, acc, za);
+                byte_copy<K, INIT, LIKE>(xb, vb, cnt0, va ? w.y : w.x, q0, s_pl, A, s_hl, T0, V0,
+                                         acc, zb);
+                byte_copy<K, INIT, LIKE>(xc, vc, cnt1, w.z, q1, s_pl, A, s_hl, T1, V1, acc, zc);
+                byte_copy<K, INIT, LIKE>(xd, vd, cnt1, vc ? w.w : w.z, q1, s_pl, A, s_hl, T1, V1,
+                                         acc, zd);
                 if (LIKE) {
                     acc.renorm();
                 }

@@ -92,6 +92,10 @@ struct Dataset {
     std::vector<int32_t> nall_host;   // NumAlleles, for the UpdateP work lists
     uint32_t *geno = nullptr;
     int32_t *nall = nullptr, *ahist = nullptr;
+    // POPFLAG of the data file (null without one): per individual, and as the
+    // per-word-row masks of the 2-bit path (bit 2t: individual 16g+t has POPFLAG 1)
+    int32_t *popflag = nullptr;
+    uint32_t *pfmask = nullptr;
     int nhist = 0, nloci_nonzero = 0;
 };
 
@@ -185,6 +189,9 @@ ModelArgs model_args(const sgpu_model &m)
     a.invsqrtn = pow((double)m.N, -0.5);
     a.seed = m.seed;
     a.chain = m.chain;
+    a.ancestdist = m.ancestdist ? 1 : 0;
+    a.numboxes = m.ancestdist ? m.numboxes : 0;
+    a.pfonly = m.pfrompopflagonly ? 1 : 0;
     return a;
 }
 
@@ -370,13 +377,17 @@ size_t state_elems(const sgpu_model &m, int which, size_t *elem)
     case SGPU_NQ: *elem = sizeof(int32_t); return N * K;
     case SGPU_LIKE: return 3;
     case SGPU_NSUM: *elem = sizeof(int64_t); return 1;
+    case SGPU_ANCESTDIST:
+        *elem = sizeof(int32_t);
+        return m.ancestdist ? N * K * (size_t)m.numboxes : 0;
     default: return 0;
     }
 }
 
-double *state_ptr(const ChainDesc &d, int which)
+void *state_ptr(const ChainDesc &d, int which)
 {
     switch (which) {
+    case SGPU_ANCESTDIST: return d.adist;
     case SGPU_Q: return d.Q;
     case SGPU_P: return d.P;
     case SGPU_EPS: return d.eps;
@@ -524,6 +535,14 @@ int check_model(const sgpu_model *m, int ci)
     if (m->freqscorr && (m->fpriormean <= 0.0 || m->fpriorsd <= 0.0)) {
         return fail(SGPU_ERR_ARG, "chain %d: FPRIORMEAN and FPRIORSD must be positive", ci);
     }
+    if (m->ancestdist && (m->numboxes < 1 || (size_t)m->N * m->K * m->numboxes > ((size_t)1 << 34))) {
+        return fail(SGPU_ERR_ARG, "chain %d: NUMBOXES=%d outside [1, 2^34 / (N K)]", ci,
+                    m->numboxes);
+    }
+    // the kernels index [L][K][A] and [N][K] with 32-bit products
+    if ((size_t)m->L * m->K * m->A >= ((size_t)1 << 31) || (size_t)m->N * m->K >= ((size_t)1 << 31)) {
+        return fail(SGPU_ERR_UNSUPPORTED, "chain %d: L*K*A or N*K reaches 2^31", ci);
+    }
     return SGPU_OK;
 }
 
@@ -612,6 +631,23 @@ int upload_dataset(sgpu_ctx *c, Dataset &ds, const sgpu_data *d)
     ds.nhist = (int)hist.size();
     ds.nloci_nonzero = ds.L - nzero;
     ds.nall_host.assign(d->nall, d->nall + ds.L);
+    if (d->popflag) {
+        const int G2 = (int)cdiv(ds.N, 16);
+        std::vector<uint32_t> masks((size_t)G2, 0u);
+        for (int i = 0; i < ds.N; i++) {
+            if (d->popflag[i] == 1) {
+                masks[(size_t)(i / 16)] |= 1u << (2 * (i % 16));
+            }
+        }
+        if ((rc = dalloc(c, &ds.popflag, ds.N)) || (rc = dalloc(c, &ds.pfmask, masks.size()))) {
+            return rc;
+        }
+        HIP_OK(hipMemcpyAsync(ds.popflag, d->popflag, (size_t)ds.N * sizeof(int32_t),
+                              hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(ds.pfmask, masks.data(), masks.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));   // masks is freed on return
+    }
     HIP_OK(hipMemcpyAsync(ds.geno, d->geno, need * sizeof(uint32_t), hipMemcpyHostToDevice,
                           c->stream));
     HIP_OK(hipMemcpyAsync(ds.nall, d->nall, (size_t)ds.L * sizeof(int32_t), hipMemcpyHostToDevice,
@@ -818,8 +854,26 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         d.n_ranges = (int)cdiv(L, d.loci_per_range);
     }
     d.zout = nullptr;
+    d.adist = nullptr;
+    d.pfmask = nullptr;
+    d.popflag = nullptr;
+    if (m.pfrompopflagonly) {
+        if (!ds.popflag) {
+            // CheckIfMissing's rule (Util.c:75-77)
+            return fail(SGPU_ERR_ARG, "PFROMPOPFLAGONLY can only be turned on when the data "
+                                      "set has POPFLAG data");
+        }
+        if (ds.fmt == SGPU_GENO_SNP2) {
+            d.pfmask = ds.pfmask;
+        } else {
+            d.popflag = ds.popflag;
+        }
+    }
 
     int rc;
+    if (m.ancestdist && (rc = dalloc(c, &d.adist, (size_t)N * K * m.numboxes))) {
+        return rc;
+    }
     if ((rc = dalloc(c, &d.Q, (size_t)N * K)) || (rc = dalloc(c, &d.P, (size_t)L * K * A)) ||
         (rc = dalloc(c, &d.eps, (size_t)L * A)) || (rc = dalloc(c, &d.F, K)) ||
         (rc = dalloc(c, &d.alpha, K)) || (rc = dalloc(c, &d.lambda, K)) ||
@@ -1168,6 +1222,10 @@ int sgpu_init_state(sgpu_ctx *c)
         HIP_OK(hipMemsetAsync(d.alphaSum, 0, K * sizeof(double), c->stream));
         HIP_OK(hipMemsetAsync(d.lambdaSum, 0, K * sizeof(double), c->stream));
         HIP_OK(hipMemsetAsync(d.like3, 0, 3 * sizeof(double), c->stream));
+        if (d.adist) {
+            HIP_OK(hipMemsetAsync(d.adist, 0, (size_t)N * K * m.numboxes * sizeof(int32_t),
+                                  c->stream));
+        }
         // the host vectors die at the end of this iteration
         HIP_OK(hipStreamSynchronize(c->stream));
     }
@@ -1198,7 +1256,10 @@ int sgpu_get_chain_state(sgpu_ctx *c, int32_t chain, int32_t which, void *host, 
     size_t elem;
     const size_t n = state_elems(h.m, which, &elem);
     if (n == 0) {
-        return fail(SGPU_ERR_ARG, "unknown state selector %d", which);
+        return fail(SGPU_ERR_ARG, which == SGPU_ANCESTDIST
+                                      ? "chain %d runs without ANCESTDIST"
+                                      : "unknown state selector %d",
+                    which == SGPU_ANCESTDIST ? chain : which);
     }
     if (bytes != n * elem) {
         return fail(SGPU_ERR_ARG, "state %d needs %zu bytes, got %zu", which, n * elem, bytes);
@@ -1236,7 +1297,10 @@ int sgpu_set_chain_state(sgpu_ctx *c, int32_t chain, int32_t which, const void *
     size_t elem;
     const size_t n = state_elems(h.m, which, &elem);
     if (n == 0) {
-        return fail(SGPU_ERR_ARG, "unknown state selector %d", which);
+        return fail(SGPU_ERR_ARG, which == SGPU_ANCESTDIST
+                                      ? "chain %d runs without ANCESTDIST"
+                                      : "unknown state selector %d",
+                    which == SGPU_ANCESTDIST ? chain : which);
     }
     if (bytes != n * elem) {
         return fail(SGPU_ERR_ARG, "state %d needs %zu bytes, got %zu", which, n * elem, bytes);

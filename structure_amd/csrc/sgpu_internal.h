@@ -33,6 +33,8 @@ struct ModelArgs {
     int N, L, K, A;
     int freqscorr, onefst, popalphas, unifprioralpha, popspecificlambda;
     int inferalpha, inferlambda, pad0;
+    int ancestdist, numboxes;      // ANCESTDIST histogram of Q (collected sweeps)
+    int pfonly, pad1;              // PFROMPOPFLAGONLY: allele counts from POPFLAG=1 only
     double alphapropsd, alphamax, alphapriora, alphapriorb, fpriormean, fpriorsd;
     double invsqrtn;   // N^-1/2 from the host's pow, the epsilon step bound (UpdateEpsilon.c:80)
     uint32_t seed, chain;
@@ -75,6 +77,12 @@ struct ChainDesc {
     double *lam_part, *fst_part;   // [l_blocks][K]
     double *QSum, *PSum, *epsSum, *FstSum, *alphaSum, *lambdaSum, *like3;
     int8_t *zout;                  // debug copy of Z [N][2][L] or null
+    int32_t *adist;                // [N][K][numboxes] ANCESTDIST histogram, or null
+    // PFROMPOPFLAGONLY (null when off): 2-bit words, per word row the bits 2t of
+    // the individuals 16g+t whose POPFLAG is 1 ([G]); byte words, POPFLAG per
+    // individual ([N])
+    const uint32_t *pfmask;
+    const int32_t *popflag;
 };
 
 #ifdef __HIP_DEVICE_COMPILE__

@@ -125,3 +125,49 @@ def test_dirichlet_updates_have_the_posterior_mean():
     mean = par / par.sum(axis=2, keepdims=True)
     sd = 0.5 / np.sqrt(par.sum(axis=2, keepdims=True) + 1) / np.sqrt(R)
     assert np.all(np.abs(acc - mean) < 5 * sd)
+
+
+def test_ancestdist_box_rule_by_brute_force():
+    """The oracle's ANCESTDIST histogram is floor(Q * NUMBOXES), clamped to the
+    last box, counted over the collected sweeps (checked without the reference)."""
+    rng = np.random.default_rng(17)
+    N, L, K, A, nb = 10, 6, 4, 3, 20
+    geno, nall = random_geno(rng, N, L, A, miss=0.0)
+    m = oracle.make_model(N, L, K, A, ancestdist=1, numboxes=nb, computeprob=0)
+    ch = oracle.Chain(m, geno, nall)
+    ch.init()
+    expect = np.zeros((N, K, nb), np.int64)
+    for s in range(6):
+        Q = random_q(rng, N, K, a=0.3)
+        Q[s % N] = np.eye(K)[s % K]
+        ch.arrays["Q"][:] = Q
+        oracle.lib().orc_collect(oracle.C.byref(m), oracle.bp(ch.geno), oracle.ip(ch.nall),
+                                 oracle.C.byref(ch.state))
+        box = np.minimum((Q * nb).astype(np.int64), nb - 1)
+        for i in range(N):
+            for k in range(K):
+                expect[i, k, box[i, k]] += 1
+    assert np.array_equal(ch.ancestDist, expect)
+
+
+def test_popflag_filter_counts_by_recount():
+    """PFROMPOPFLAGONLY in the oracle's UpdateZ: the allele counts equal a
+    recount of the debug Z over the individuals with POPFLAG 1, the ancestry
+    counts a recount over all (checked without the reference)."""
+    rng = np.random.default_rng(23)
+    N, L, K, A = 30, 25, 3, 4
+    geno, nall = random_geno(rng, N, L, A, miss=0.05)
+    flags = (rng.random(N) < 0.5).astype(np.int32)
+    m = oracle.make_model(N, L, K, A, pfrompopflagonly=1, seed=3)
+    ch = oracle.Chain(m, geno, nall, popflag=flags)
+    nA, nQ, Z = ch.update_z(random_q(rng, N, K), random_p(rng, L, K, A, nall), 2, want_z=True)
+    expect = np.zeros_like(nA)
+    for i in range(N):
+        if flags[i] != 1:
+            continue
+        for c in range(2):
+            for l in range(L):
+                if geno[i, c, l] != -9:
+                    expect[l, Z[i, c, l], geno[i, c, l]] += 1
+    assert np.array_equal(nA, expect)
+    assert np.array_equal(nQ.sum(axis=1), (geno != -9).sum(axis=(1, 2)))

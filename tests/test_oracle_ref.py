@@ -479,3 +479,81 @@ def test_est_log_prob_formula(ref):
     mean = s / n
     var = (ss - s * s / n) / (n - 1)
     assert ref.lib().EstLogProb(s, ss, n) == mean - var / 2.0
+
+
+def _individuals_with_flags(ref, flags):
+    """struct IND array (structure.h:52-59) with PopFlag set per individual."""
+    N = len(flags)
+    inds = ref.individuals(N)
+    buf = bytearray(ref.IND_SIZE * N)
+    for i, f in enumerate(flags):
+        buf[i * ref.IND_SIZE + 16:i * ref.IND_SIZE + 20] = int(f).to_bytes(4, "little",
+                                                                          signed=True)
+    import ctypes
+    ctypes.memmove(inds, bytes(buf), len(buf))
+    return inds
+
+
+@pytest.mark.parametrize("K,A", [(3, 5), (2, 2)])
+def test_popflag_only_counts_match_reference(ref, K, A):
+    """PFROMPOPFLAGONLY: GetNumFromPops (UpdateP.c:60-99) counts only individuals
+    whose PopFlag is 1 (values 0, 1 and 2 present); the ancestry counts keep all."""
+    rng = np.random.default_rng(300 + K)
+    N, L = 21, 17
+    geno, nall = random_geno(rng, N, L, A, miss=0.05)
+    flags = rng.integers(0, 3, size=N).astype(np.int32)
+    flags[:3] = [1, 0, 2]
+    Q = random_q(rng, N, K)
+    P = random_p(rng, L, K, A, nall)
+    rep, seed, chain = 4, 99, 1
+    ref.configure(N, L, K, A, PFROMPOPFLAGONLY=1)
+    g32 = geno_ref_layout(geno)
+    Zr = np.zeros((N, 2, L), np.int32)
+    ref.lib().UpdateZ(ref.ip(Zr), ref.dp(Q), ref.dp(P), ref.ip(g32),
+                      ref.dp(z_uniforms(seed, chain, rep, N, L)))
+    nAr = np.zeros((L, K, A), np.int32)
+    ref.lib().GetNumFromPops(ref.ip(nAr), ref.ip(g32), ref.ip(Zr), ref.ip(nall),
+                             _individuals_with_flags(ref, flags))
+    ref.configure(N, L, K, A)   # back to the default for the other tests
+    m = oracle.make_model(N, L, K, A, seed=seed, chain=chain, pfrompopflagonly=1)
+    ch = oracle.Chain(m, geno, nall, popflag=flags)
+    nA, nQ, Z = ch.update_z(Q, P, rep, want_z=True)
+    assert np.array_equal(Z, Zr)
+    assert np.array_equal(nA, nAr)
+    assert nA.sum() < (geno != -9).sum()                 # the filter removed copies
+    assert nQ.sum() == (geno != -9).sum()                # ancestry counts keep them all
+    nA2 = np.zeros_like(nA)
+    nQ2 = np.zeros_like(nQ)
+    oracle.lib().orc_count_from_z(oracle.C.byref(m), oracle.bp(ch.geno), oracle.ip(Zr),
+                                  oracle.ip(nA2), oracle.ip(nQ2))
+    assert np.array_equal(nA2, nAr)
+
+
+def test_ancestdist_histogram_matches_reference(ref):
+    """ANCESTDIST: UpdateSums (output.c:691-700) boxes every Q[i,k] among
+    NUMBOXES boxes, Q = 1 into the last; the oracle's collection step gives the
+    same histogram over several sweeps, including the edge values 0 and 1."""
+    rng = np.random.default_rng(404)
+    N, L, K, A, nb = 12, 9, 3, 4, 7
+    geno, nall = random_geno(rng, N, L, A, miss=0.1)
+    ref.configure(N, L, K, A, ANCESTDIST=1, NUMBOXES=nb)
+    m = oracle.make_model(N, L, K, A, ancestdist=1, numboxes=nb, computeprob=0)
+    ch = oracle.Chain(m, geno, nall)
+    ch.init()
+    ad = np.zeros((N, K, nb), np.int32)
+    sums = {k: np.zeros_like(ch.arrays[k]) for k in ("QSum", "PSum", "FstSum", "epsSum",
+                                                     "lambdaSum")}
+    for sweep in range(5):
+        Q = random_q(rng, N, K, a=0.4)
+        if sweep == 0:
+            Q[0] = [1.0, 0.0, 0.0]   # the Q = 1.000 and Q = 0 edges
+        ch.arrays["Q"][:] = Q
+        ref.lib().UpdateSums(ref.dp(ch.Q), ref.dp(sums["QSum"]), None, ref.dp(ch.P),
+                             ref.dp(sums["PSum"]), ref.dp(ch.F), ref.dp(sums["FstSum"]),
+                             ref.ip(nall), ref.ip(ad), ref.dp(ch.eps), ref.dp(sums["epsSum"]),
+                             ref.dp(ch.lambda_), ref.dp(sums["lambdaSum"]), None, None, 0)
+        oracle.lib().orc_collect(oracle.C.byref(m), oracle.bp(ch.geno), oracle.ip(nall),
+                                 oracle.C.byref(ch.state))
+    ref.configure(N, L, K, A)
+    assert np.array_equal(ch.ancestDist, ad)
+    assert ad.sum() == 5 * N * K and ad[0, 0, nb - 1] >= 1 and ad[0, 1, 0] >= 1
