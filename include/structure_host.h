@@ -100,7 +100,7 @@ void shost_free_input(shost_input *in);
  * sgpu_pack_snp2 / sgpu_pack_byte.  out holds N*2*L bytes. */
 int shost_geno_int8(const shost_input *in, int8_t *out);
 
-/* Posterior sums of a finished run (device state after the last sweep). */
+/* Posterior sums of a run (device state after the last sweep so far). */
 typedef struct {
     int32_t rep;               /* BURNIN + number of collected sweeps */
     const double *qsum;        /* [N][K] */
@@ -108,12 +108,14 @@ typedef struct {
     const double *epssum;      /* [L][A] (F model) */
     const double *fstsum, *alphasum, *lambdasum;   /* [K] */
     double sumlikes, sumsqlikes;
+    const int32_t *ancestdist; /* [N][K][NUMBOXES] with ANCESTDIST, else NULL */
+    int32_t saveindex;         /* final = 0: (rep - BURNIN) / savefreq, the file's number */
 } shost_results;
 
 /* OutPutResults (output.c:1354-1423): final = 1 writes <OUTFILE>_f, the
  * stdout summary and, with PRINTQHAT, the <OUTFILE>_q file (PrintQFile,
  * output.c:915-953); final = 2 writes <OUTFILE>_f only; final = 0 writes the
- * intermediate file <OUTFILE>_<collected sweeps>. */
+ * intermediate file <OUTFILE>_<saveindex> of INTERMEDSAVE (structure.c:640-661). */
 int shost_write_results(const shost_params *p, const shost_input *in, const shost_results *r,
                         int final, int argc, char **argv);
 
@@ -130,6 +132,15 @@ void shost_print_q(FILE *f, const shost_params *p, const shost_input *in,
 void shost_print_p(FILE *f, const shost_params *p, const shost_input *in,
                    const shost_results *r);                        /* output.c:956-1005 */
 double shost_est_log_prob(double sumlikes, double sumsqlikes, int reps);   /* output.c:457-467 */
+
+/* INTERMEDSAVE's save interval (Init.c:377-385): NUMREPS / (INTERMEDSAVE + 1),
+ * rounded up; 0 without INTERMEDSAVE. */
+int32_t shost_save_freq(const shost_params *p);
+
+/* CalcLike (CalcLike.c:387-426) of (Q [N][K], P [L][K][A]) on the recoded
+ * genotypes, on the host: PRINTLIKES prints it for burn-in sweeps, whose lnP(D)
+ * the sweep does not accumulate (PrintLike, output.c:187-200). */
+double shost_calc_like(const shost_input *in, int K, const double *Q, const double *P);
 
 #ifdef __cplusplus
 }

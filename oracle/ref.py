@@ -243,9 +243,11 @@ def read_input(params: dict):
 
 
 def output_results(params: dict, inp: dict, rep, qsum, psum, epssum, fstsum, alphasum,
-                   lambdasum, sumlikes, sumsqlikes, args=("structure",)) -> None:
-    """OutPutResults(final=1) (output.c:1354-1423) for the state given;
-    writes <OUTFILE>_f (and _q with PRINTQHAT)."""
+                   lambdasum, sumlikes, sumsqlikes, args=("structure",), ancestdist=None,
+                   final=1, savefreq=0) -> None:
+    """OutPutResults (output.c:1354-1423) for the state given; final=1 writes
+    <OUTFILE>_f (and _q with PRINTQHAT), final=0 the intermediate file
+    <OUTFILE>_<(rep - BURNIN) / savefreq>."""
     L = lib()
     set_params_globals(params)
     set_int("MAXALLELES", inp["maxalleles"])
@@ -282,7 +284,9 @@ def output_results(params: dict, inp: dict, rep, qsum, psum, epssum, fstsum, alp
     lam = np.ascontiguousarray(lambdasum, dtype=np.float64)
     n, argv = _argv(list(args))
     psum_, qsum_, fst_, alpha_, eps_, lsum_ = arrs
-    L.OutPutResults(ip(geno), rep, 0, inds, dp(psum_), dp(qsum_), dp(fst_), None, None,
+    ad = None if ancestdist is None else np.ascontiguousarray(ancestdist, dtype=np.int32)
+    L.OutPutResults(ip(geno), rep, savefreq, inds, dp(psum_), dp(qsum_), dp(fst_),
+                    None if ad is None else ip(ad), None,
                     sumlikes, sumsqlikes, dp(alpha_), dp(zeros), dp(zeros), ip(nall), ip(trans),
-                    1, marker, dp(zeros), dp(eps_), dp(lam), dp(lsum_), dp(zeros), 0,
+                    final, marker, dp(zeros), dp(eps_), dp(lam), dp(lsum_), dp(zeros), 0,
                     dp(zeros), dp(zeros), n, argv)

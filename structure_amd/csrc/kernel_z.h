@@ -97,7 +97,7 @@ struct LikeAcc {
 // Q[i,:] (wave uniform) is read through the constant address space, so it is
 // fetched with scalar loads.
 // ---------------------------------------------------------------------------
-template <int K, bool INIT, bool LIKE, bool ZOUT>
+template <int K, bool INIT, bool LIKE, bool ZOUT, bool PF>
 __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__restrict__ cds,
                                                           const BlockRef *__restrict__ map,
                                                           const uint32_t *__restrict__ repp)
@@ -119,8 +119,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     SGPU_G uint16_t *nQpart = g(C.nQpart);
     SGPU_G double *likepart = g(C.likepart);
     SGPU_G int8_t *zout = g(C.zout);
-    // PFROMPOPFLAGONLY: per word row, bit 2t set when individual 16g+t adds to the
-    // allele counts (null: every individual does)
+    // PFROMPOPFLAGONLY (PF): per word row, bit 2t set when individual 16g+t adds to
+    // the allele counts
     const SGPU_G uint32_t *pfmask = g(C.pfmask);
     extern __shared__ uint32_t s_cnt[];   // [tile individual][K]: valid copies, then z > k
     const int lane = threadIdx.x & 63;
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     uint32_t word = geno[(size_t)row0 * Lpad + l];
     for (int g = row0; g < row1; g++) {
         const uint32_t next = (g + 1 < row1) ? geno[(size_t)(g + 1) * Lpad + l] : 0u;
-        const uint32_t pfm = pfmask ? pfmask[g] : 0x55555555u;   // wave-uniform
+        const uint32_t pfm = PF ? pfmask[g] : 0x55555555u;   // wave-uniform
         {
             // this row's valid copies and allele-1 copies at the lane's locus, from the
             // word's bit pattern (code 3 = missing; only individuals < N count)
@@ -196,9 +196,9 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                 const bool x0 = code == 2u;
                 const bool x1 = code != 0u;
                 const uint32_t X0 = x0 ? 0x10001u : 1u, X1 = x1 ? 0x10001u : 1u;
-                // lanes whose copies go into the allele counts: all, or none when the
-                // individual's POPFLAG is not 1 (its ancestry counts still take them)
-                const uint64_t fm = ((pfm >> (2 * t)) & 1u) ? ~0ull : 0ull;
+                // lanes whose copies go into the allele counts: all, or (PF) none when
+                // the individual's POPFLAG is not 1 (its ancestry counts still take them)
+                const uint64_t fm = (!PF || ((pfm >> (2 * t)) & 1u)) ? ~0ull : 0ull;
                 const uint32_t wa = h ? w.z : w.x;
                 const uint32_t wb = h ? w.w : w.y;
                 const uint64_t vm = ballot(valid);
@@ -344,7 +344,7 @@ constexpr int kByteRows = 256;   // most word rows (individual pairs) per Update
 
 // One allele copy: its draw, its "z > k" counts (T, V) and, when `count`, its
 // bin of the locus's allele-by-population histogram (s_hl).
-template <int K, bool INIT, bool LIKE>
+template <int K, bool INIT, bool LIKE, bool PF>
 __device__ __forceinline__ void byte_copy(int x, bool valid, bool count, uint32_t w,
                                           const double *q, const double *s_pl, int A,
                                           uint32_t *s_hl, uint32_t *T, uint32_t &V, LikeAcc &acc,
@@ -387,13 +387,13 @@ __device__ __forceinline__ void byte_copy(int x, bool valid, bool count, uint32_
             z += gt;
         }
     }
-    if (count) {
+    if (!PF || count) {
         atomicAdd(&s_hl[z * A + x], 1u);
     }
     zout = z;
 }
 
-template <int K, bool INIT, bool LIKE, bool ZOUT>
+template <int K, bool INIT, bool LIKE, bool ZOUT, bool PF>
 __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__restrict__ cds,
                                                              const BlockRef *__restrict__ map,
                                                              const uint32_t *__restrict__ repp)
@@ -421,10 +421,10 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
     const bool rowok = h < G;
     const int i0 = 2 * h, i1 = 2 * h + 1;
     const bool ok0 = rowok && i0 < N, ok1 = rowok && i1 < N;
-    // PFROMPOPFLAGONLY: only individuals with POPFLAG 1 add to the allele counts
+    // PFROMPOPFLAGONLY (PF): only individuals with POPFLAG 1 add to the allele counts
     const SGPU_G int32_t *popflag = g(C.popflag);
-    const bool cnt0 = !popflag || (ok0 && popflag[i0] == 1);
-    const bool cnt1 = !popflag || (ok1 && popflag[i1] == 1);
+    const bool cnt0 = !PF || (ok0 && popflag[i0] == 1);
+    const bool cnt1 = !PF || (ok1 && popflag[i1] == 1);
     double q0[K], q1[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -476,11 +476,11 @@ __global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__
                 const bool va = ok0 && xa != 0xff, vb = ok0 && xb != 0xff;
                 const bool vc = ok1 && xc != 0xff, vd = ok1 && xd != 0xff;
                 int za = -9, zb = -9, zc = -9, zd = -9;
-                byte_copy<K, INIT, LIKE>(xa, va, cnt0, w.x, q0, s_pl, A, s_hl, T0, V0, acc, za);
-                byte_copy<K, INIT, LIKE>(xb, vb, cnt0, va ? w.y : w.x, q0, s_pl, A, s_hl, T0, V0,
+                byte_copy<K, INIT, LIKE, PF>(xa, va, cnt0, w.x, q0, s_pl, A, s_hl, T0, V0, acc, za);
+                byte_copy<K, INIT, LIKE, PF>(xb, vb, cnt0, va ? w.y : w.x, q0, s_pl, A, s_hl, T0, V0,
                                          acc, zb);
-                byte_copy<K, INIT, LIKE>(xc, vc, cnt1, w.z, q1, s_pl, A, s_hl, T1, V1, acc, zc);
-                byte_copy<K, INIT, LIKE>(xd, vd, cnt1, vc ? w.w : w.z, q1, s_pl, A, s_hl, T1, V1,
+                byte_copy<K, INIT, LIKE, PF>(xc, vc, cnt1, w.z, q1, s_pl, A, s_hl, T1, V1, acc, zc);
+                byte_copy<K, INIT, LIKE, PF>(xd, vd, cnt1, vc ? w.w : w.z, q1, s_pl, A, s_hl, T1, V1,
                                          acc, zd);
                 if (LIKE) {
                     acc.renorm();

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "../../include/sgpu.h"
@@ -81,9 +82,11 @@ struct Plan {
     size_t lds = 0;
 };
 
-// A launch of a K-templated kernel: the chains of one (format, K).
+// A launch of a K-templated kernel: the chains of one (format, K) and, for
+// UpdateZ, one PFROMPOPFLAGONLY setting.
 struct KGroup {
     int fmt = 0, K = 0;
+    bool pf = false;
     Plan plan;
 };
 
@@ -250,51 +253,58 @@ auto all_chains = [](const HostChain &) { return true; };
 
 typedef void (*zkern_t)(const ChainDesc *, const BlockRef *, const uint32_t *);
 
-template <int K>
-zkern_t pick_z(int fmt, int mode)
+template <int K, bool PF>
+zkern_t pick_z_pf(int fmt, int mode)
 {
     if (fmt == SGPU_GENO_BYTE) {
         switch (mode) {
-        case 0: return k_update_z_byte<K, true, false, false>;
-        case 1: return k_update_z_byte<K, false, false, false>;
-        case 2: return k_update_z_byte<K, false, true, false>;
-        case 3: return k_update_z_byte<K, false, false, true>;
-        default: return k_update_z_byte<K, false, true, true>;
+        case 0: return k_update_z_byte<K, true, false, false, PF>;
+        case 1: return k_update_z_byte<K, false, false, false, PF>;
+        case 2: return k_update_z_byte<K, false, true, false, PF>;
+        case 3: return k_update_z_byte<K, false, false, true, PF>;
+        default: return k_update_z_byte<K, false, true, true, PF>;
         }
     }
     switch (mode) {
-    case 0: return k_update_z_snp2<K, true, false, false>;
-    case 1: return k_update_z_snp2<K, false, false, false>;
-    case 2: return k_update_z_snp2<K, false, true, false>;
-    case 3: return k_update_z_snp2<K, false, false, true>;
-    default: return k_update_z_snp2<K, false, true, true>;
+    case 0: return k_update_z_snp2<K, true, false, false, PF>;
+    case 1: return k_update_z_snp2<K, false, false, false, PF>;
+    case 2: return k_update_z_snp2<K, false, true, false, PF>;
+    case 3: return k_update_z_snp2<K, false, false, true, PF>;
+    default: return k_update_z_snp2<K, false, true, true, PF>;
     }
 }
 
+// PF: the variant that filters the allele counts by POPFLAG (PFROMPOPFLAGONLY)
+template <int K>
+zkern_t pick_z(int fmt, int mode, bool pf)
+{
+    return pf ? pick_z_pf<K, true>(fmt, mode) : pick_z_pf<K, false>(fmt, mode);
+}
+
 // mode 0 = initial Z, 1 = update, 2 = update + lnP terms (+2 with a debug copy of Z)
-zkern_t z_kernel(int fmt, int K, int mode)
+zkern_t z_kernel(int fmt, int K, int mode, bool pf)
 {
     switch (K) {
-    case 1: return pick_z<1>(fmt, mode);
-    case 2: return pick_z<2>(fmt, mode);
-    case 3: return pick_z<3>(fmt, mode);
-    case 4: return pick_z<4>(fmt, mode);
-    case 5: return pick_z<5>(fmt, mode);
-    case 6: return pick_z<6>(fmt, mode);
-    case 7: return pick_z<7>(fmt, mode);
-    case 8: return pick_z<8>(fmt, mode);
-    case 9: return pick_z<9>(fmt, mode);
-    case 10: return pick_z<10>(fmt, mode);
-    case 11: return pick_z<11>(fmt, mode);
-    case 12: return pick_z<12>(fmt, mode);
-    case 13: return pick_z<13>(fmt, mode);
-    case 14: return pick_z<14>(fmt, mode);
-    case 15: return pick_z<15>(fmt, mode);
-    case 16: return pick_z<16>(fmt, mode);
-    case 17: return pick_z<17>(fmt, mode);
-    case 18: return pick_z<18>(fmt, mode);
-    case 19: return pick_z<19>(fmt, mode);
-    case 20: return pick_z<20>(fmt, mode);
+    case 1: return pick_z<1>(fmt, mode, pf);
+    case 2: return pick_z<2>(fmt, mode, pf);
+    case 3: return pick_z<3>(fmt, mode, pf);
+    case 4: return pick_z<4>(fmt, mode, pf);
+    case 5: return pick_z<5>(fmt, mode, pf);
+    case 6: return pick_z<6>(fmt, mode, pf);
+    case 7: return pick_z<7>(fmt, mode, pf);
+    case 8: return pick_z<8>(fmt, mode, pf);
+    case 9: return pick_z<9>(fmt, mode, pf);
+    case 10: return pick_z<10>(fmt, mode, pf);
+    case 11: return pick_z<11>(fmt, mode, pf);
+    case 12: return pick_z<12>(fmt, mode, pf);
+    case 13: return pick_z<13>(fmt, mode, pf);
+    case 14: return pick_z<14>(fmt, mode, pf);
+    case 15: return pick_z<15>(fmt, mode, pf);
+    case 16: return pick_z<16>(fmt, mode, pf);
+    case 17: return pick_z<17>(fmt, mode, pf);
+    case 18: return pick_z<18>(fmt, mode, pf);
+    case 19: return pick_z<19>(fmt, mode, pf);
+    case 20: return pick_z<20>(fmt, mode, pf);
     default: return nullptr;
     }
 }
@@ -337,7 +347,7 @@ mkern_t metro_kernel(int fmt, int K)
 int launch_z(sgpu_ctx *c, int mode)
 {
     for (const KGroup &g : c->z_groups) {
-        zkern_t k = z_kernel(g.fmt, g.K, mode);
+        zkern_t k = z_kernel(g.fmt, g.K, mode, g.pf);
         if (!k) {
             return fail(SGPU_ERR_UNSUPPORTED, "no UpdateZ kernel for K=%d", g.K);
         }
@@ -949,17 +959,24 @@ int build_plans(sgpu_ctx *c)
             return rc;
         }
     }
-    // K-templated launches: one per (format, K), heavier K first
-    std::map<std::pair<int, int>, int> keys;
+    // K-templated launches, heavier K first: UpdateZ once per (format, K,
+    // PFROMPOPFLAGONLY), the Metropolis ratio once per (format, K)
+    std::map<std::tuple<int, int, int>, int> zkeys;
+    std::map<std::pair<int, int>, int> mkeys;
     for (const HostChain &h : c->chains) {
-        keys[{-h.d.fmt, -h.m.K}] = 1;
+        zkeys[{-h.d.fmt, -h.m.K, h.m.pfrompopflagonly ? 1 : 0}] = 1;
+        mkeys[{-h.d.fmt, -h.m.K}] = 1;
     }
-    for (auto &kv : keys) {
-        const int fmt = -kv.first.first, K = -kv.first.second;
-        auto sel = [fmt, K](const HostChain &h) { return h.d.fmt == fmt && h.m.K == K; };
+    for (auto &kv : zkeys) {
+        const int fmt = -std::get<0>(kv.first), K = -std::get<1>(kv.first);
+        const bool pf = std::get<2>(kv.first) != 0;
+        auto sel = [fmt, K, pf](const HostChain &h) {
+            return h.d.fmt == fmt && h.m.K == K && (h.m.pfrompopflagonly != 0) == pf;
+        };
         KGroup zg;
         zg.fmt = fmt;
         zg.K = K;
+        zg.pf = pf;
         // x fastest, so the blocks running at one time share their individual
         // tile.  Orders that keep a locus tile's blocks on one XCD (y fastest, or
         // XCD bands of locus tiles) fetch each frequency tile once per XCD
@@ -975,6 +992,10 @@ int build_plans(sgpu_ctx *c)
             }
         }
         c->z_groups.push_back(zg);
+    }
+    for (auto &kv : mkeys) {
+        const int fmt = -kv.first.first, K = -kv.first.second;
+        auto sel = [fmt, K](const HostChain &h) { return h.d.fmt == fmt && h.m.K == K; };
         KGroup mg;
         mg.fmt = fmt;
         mg.K = K;
@@ -986,8 +1007,6 @@ int build_plans(sgpu_ctx *c)
         }
         c->metro_groups.push_back(mg);
     }
-    // byte chains of different K share one runtime-K kernel, but each group still
-    // launches separately; that keeps the LDS size per launch tight
     return SGPU_OK;
 }
 

@@ -67,7 +67,8 @@ class Results(C.Structure):
     _fields_ = [("rep", _I), ("qsum", C.POINTER(_D)), ("psum", C.POINTER(_D)),
                 ("epssum", C.POINTER(_D)), ("fstsum", C.POINTER(_D)),
                 ("alphasum", C.POINTER(_D)), ("lambdasum", C.POINTER(_D)),
-                ("sumlikes", _D), ("sumsqlikes", _D)]
+                ("sumlikes", _D), ("sumsqlikes", _D), ("ancestdist", C.POINTER(_I)),
+                ("saveindex", _I)]
 
 
 _SIGS = {
@@ -85,6 +86,8 @@ _SIGS = {
     "shost_write_results": (C.c_int, [C.POINTER(Params), C.POINTER(Input), C.POINTER(Results),
                                       C.c_int, C.c_int, C.POINTER(C.c_char_p)]),
     "shost_est_log_prob": (_D, [_D, _D, C.c_int]),
+    "shost_save_freq": (_I, [C.POINTER(Params)]),
+    "shost_calc_like": (_D, [C.POINTER(Input), C.c_int, C.c_void_p, C.c_void_p]),
     "shost_assign": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "shost_align_q": (C.c_int, [C.c_void_p, C.c_void_p, _I, _I, C.c_void_p]),
     "shost_permute_clusters": (C.c_int, [C.c_void_p, _I, _I, _I, C.c_void_p, C.c_void_p]),
@@ -182,16 +185,25 @@ class InputFile:
         return out
 
     def write_results(self, rep, qsum, psum, epssum, fstsum, alphasum, lambdasum, sumlikes,
-                      sumsqlikes, final=1, args=("structure",), params: Params | None = None):
+                      sumsqlikes, final=1, args=("structure",), params: Params | None = None,
+                      ancestdist=None, saveindex=0):
         p = params if params is not None else self.params
         keep = [np.ascontiguousarray(a, dtype=np.float64)
                 for a in (qsum, psum, epssum, fstsum, alphasum, lambdasum)]
         ptr = [a.ctypes.data_as(C.POINTER(_D)) for a in keep]
-        r = Results(rep, *ptr, sumlikes, sumsqlikes)
+        ad = None if ancestdist is None else np.ascontiguousarray(ancestdist, dtype=np.int32)
+        r = Results(rep, *ptr, sumlikes, sumsqlikes,
+                    None if ad is None else ad.ctypes.data_as(C.POINTER(_I)), saveindex)
         n, arr = _argv(list(args))
         rc = _lib().shost_write_results(C.byref(p), C.byref(self._in), C.byref(r), final, n, arr)
         if rc:
             raise SgpuError(rc, "cannot write results")
+
+    def calc_like(self, Q, P) -> float:
+        """CalcLike (CalcLike.c:387-426) on the host, as PRINTLIKES uses it."""
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        return _lib().shost_calc_like(C.byref(self._in), Q.shape[1], _vp(Q), _vp(P))
 
     def close(self):
         if self._in is not None:
@@ -207,6 +219,11 @@ class InputFile:
 
 def est_log_prob(sumlikes, sumsqlikes, reps) -> float:
     return _lib().shost_est_log_prob(sumlikes, sumsqlikes, reps)
+
+
+def save_freq(p: Params) -> int:
+    """INTERMEDSAVE's interval between intermediate result files (Init.c:377-385)."""
+    return _lib().shost_save_freq(C.byref(p))
 
 
 def assign(cost: np.ndarray) -> np.ndarray:

@@ -1,10 +1,11 @@
-/* Result files of the `structure` command: <OUTFILE>_f and <OUTFILE>_q.
+/* Result files of the `structure` command: <OUTFILE>_f, <OUTFILE>_q and the
+ * INTERMEDSAVE files <OUTFILE>_<n>.
  *
  * The text is the reference's (OutPutResults, output.c:1354-1423, and the
  * printers it calls), so scripts that parse STRUCTURE output read these files
- * unchanged.  Only the admixture / F-model sections are produced (no
- * USEPOPINFO, LINKAGE, LOCPRIOR or ANCESTDIST columns: those models are not
- * part of the sweep). */
+ * unchanged.  Only the admixture / F-model sections are produced, with the
+ * ANCESTDIST probability intervals (no USEPOPINFO, LINKAGE or LOCPRIOR
+ * columns: those models are not part of the sweep). */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -322,6 +323,31 @@ static int missing_loc(const shost_input *in, int l)
     return n;
 }
 
+/* PrintAncestDist (output.c:762-797): per pop the box edges where the running
+ * count of the histogram first exceeds (int)(n (1 - ANCESTPINT) / 2) and then
+ * n (1 + ANCESTPINT) / 2, n the collected sweeps. */
+static void print_ancest_dist(FILE *f, const shost_params *p, const int32_t *adist, int i, int n)
+{
+    const int K = p->maxpops, nb = p->numboxes;
+    fprintf(f, "    ");
+    for (int k = 0; k < K; k++) {
+        const int32_t *h = adist + ((size_t)i * K + k) * nb;
+        double sum = 0;
+        int low = 0;
+        for (int box = 0; box < nb; box++) {
+            sum += h[box];
+            if (!low && sum > (int)(n * (1.0 - p->ancestpint) / 2.0)) {
+                fprintf(f, "(%1.3f,", (double)(box + 0.0) / nb);
+                low = 1;
+            }
+            if (sum > (n * (1 + p->ancestpint) / 2.0)) {
+                fprintf(f, "%1.3f) ", (double)(box + 1.0) / nb);
+                break;
+            }
+        }
+    }
+}
+
 void shost_print_q(FILE *f, const shost_params *p, const shost_input *in,
                    const shost_results *r)
 {
@@ -339,6 +365,9 @@ void shost_print_q(FILE *f, const shost_params *p, const shost_input *in,
     }
     fprintf(f, ":  ");
     fprintf(f, "Inferred clusters");
+    if (p->ancestdist) {
+        fprintf(f, " (and %d%c probability intervals)", (int)(100.0 * p->ancestpint), '%');
+    }
     fprintf(f, "\n");
     for (int i = 0; i < in->N; i++) {
         const int miss = (int)(100 * missing_ind(in, i)) / (in->lines * in->L);
@@ -357,6 +386,9 @@ void shost_print_q(FILE *f, const shost_params *p, const shost_input *in,
         fprintf(f, ":  ");
         for (int k = 0; k < K; k++) {
             fprintf(f, "%1.3f ", r->qsum[(size_t)i * K + k] / n);
+        }
+        if (p->ancestdist && r->ancestdist) {
+            print_ancest_dist(f, p, r->ancestdist, i, n);
         }
         fprintf(f, "\n");
     }
@@ -446,7 +478,7 @@ int shost_write_results(const shost_params *p, const shost_input *in, const shos
     if (final) {
         snprintf(name, sizeof(name), "%s_f", p->outfile);
     } else {
-        snprintf(name, sizeof(name), "%s_%d", p->outfile, r->rep - p->burnin);
+        snprintf(name, sizeof(name), "%s_%d", p->outfile, r->saveindex);
     }
     FILE *f = fopen(name, "w");
     if (!f) {
@@ -478,4 +510,45 @@ int shost_write_results(const shost_params *p, const shost_input *in, const shos
     }
     fclose(f);
     return SGPU_OK;
+}
+
+int32_t shost_save_freq(const shost_params *p)
+{
+    if (p->intermedsave <= 0) {
+        return 0;
+    }
+    int32_t f = p->numreps / (p->intermedsave + 1);
+    if (f * (p->intermedsave + 1) < p->numreps) {   /* one save too many otherwise */
+        f++;
+    }
+    return f;
+}
+
+double shost_calc_like(const shost_input *in, int K, const double *Q, const double *P)
+{
+    const int N = in->N, L = in->L, A = in->maxalleles > 0 ? in->maxalleles : 1;
+    const double clamp = 1e-50;   /* sqrt(UNDERFLO), CalcLike.c:99 */
+    double loglike = 0.0;
+    for (int i = 0; i < N; i++) {
+        double run = 1.0, ind = 0.0;
+        for (int c = 0; c < in->lines && c < 2; c++) {
+            for (int l = 0; l < L; l++) {
+                const int x = in->geno[((size_t)i * in->lines + c) * L + l];
+                if (x == in->missing || x < 0 || x >= in->nall[l]) {
+                    continue;
+                }
+                double term = 0.0;
+                for (int k = 0; k < K; k++) {
+                    term += Q[(size_t)i * K + k] * P[((size_t)l * K + k) * A + x];
+                }
+                run *= term > clamp ? term : clamp;
+                if (run < clamp) {
+                    ind += log(run);
+                    run = 1.0;
+                }
+            }
+        }
+        loglike += ind + log(run);
+    }
+    return loglike;
 }

@@ -33,7 +33,7 @@ static int unsupported(const shost_params *p)
         const char *what;
     } u[] = {{p->linkage, "LINKAGE"}, {p->usepopinfo, "USEPOPINFO"}, {p->locprior, "LOCPRIOR"},
              {p->noadmix, "NOADMIX"}, {p->recessivealleles, "RECESSIVEALLELES"},
-             {p->ploidy != 2, "PLOIDY other than 2"}, {p->pfrompopflagonly, "PFROMPOPFLAGONLY"}};
+             {p->ploidy != 2, "PLOIDY other than 2"}};
     int bad = 0;
     for (size_t i = 0; i < sizeof(u) / sizeof(u[0]); i++) {
         if (u[i].on) {
@@ -43,6 +43,15 @@ static int unsupported(const shost_params *p)
     }
     if (p->maxpops < 1) {
         printf("Need to set MAXPOPS>0 (mainparams or -K)\n");
+        bad = 1;
+    }
+    if (p->pfrompopflagonly && !p->popflag) {   /* CheckParamCombinations, Util.c:75-78 */
+        printf("PFROMPOPFLAGONLY can only be turned on when the data file contains POPFLAG "
+               "data\n");
+        bad = 1;
+    }
+    if (p->ancestdist && p->numboxes < 1) {
+        printf("NUMBOXES must be at least 1 with ANCESTDIST\n");
         bad = 1;
     }
     return bad;
@@ -77,6 +86,9 @@ static sgpu_model model_of(const shost_params *p, int K, int A, uint32_t seed, u
     m.fpriorsd = p->fpriorsd;
     m.seed = seed;
     m.chain = chain;
+    m.ancestdist = p->ancestdist ? 1 : 0;
+    m.numboxes = p->numboxes;
+    m.pfrompopflagonly = p->pfrompopflagonly ? 1 : 0;
     return m;
 }
 
@@ -115,6 +127,7 @@ static int pack(const shost_input *in, sgpu_data *d, uint32_t **words, int *A)
     d->geno = *words;
     d->geno_words = nw;
     d->nall = in->nall;
+    d->popflag = in->popflag;   /* read by PFROMPOPFLAGONLY (POPFLAG=1 files only) */
     printf("Genotypes packed as %s (%zu words)\n", fmt == SGPU_GENO_SNP2 ? "2-bit SNP" : "bytes",
            nw);
     return rc;
@@ -124,10 +137,13 @@ static int pack(const shost_input *in, sgpu_data *d, uint32_t **words, int *A)
 typedef struct {
     double *qsum, *psum, *epssum, *fstsum, *alphasum, *lambdasum, like[3];
     double *alpha, *fst;
+    int32_t *adist;   /* ANCESTDIST histogram, or NULL */
 } sums_t;
 
 static void free_sums(sums_t *s)
 {
+    free(s->adist);
+    s->adist = NULL;
     free(s->qsum);
     free(s->psum);
     free(s->epssum);
@@ -166,7 +182,61 @@ static int read_sums(sgpu_ctx *ctx, int c, const sgpu_model *m, sums_t *s)
         (rc = sgpu_get_chain_state(ctx, c, SGPU_LIKE, s->like, sizeof(s->like)))) {
         return rc;
     }
+    if (m->ancestdist) {
+        const size_t nb = N * K * (size_t)m->numboxes;
+        s->adist = (int32_t *)malloc(sizeof(int32_t) * nb);
+        if (!s->adist) {
+            return SGPU_ERR_NOMEM;
+        }
+        if ((rc = sgpu_get_chain_state(ctx, c, SGPU_ANCESTDIST, s->adist, sizeof(int32_t) * nb))) {
+            return rc;
+        }
+    }
     return SGPU_OK;
+}
+
+static shost_results results_of(int rep, const sums_t *s)
+{
+    shost_results r;
+    memset(&r, 0, sizeof(r));
+    r.rep = rep;
+    r.qsum = s->qsum;
+    r.psum = s->psum;
+    r.epssum = s->epssum;
+    r.fstsum = s->fstsum;
+    r.alphasum = s->alphasum;
+    r.lambdasum = s->lambdasum;
+    r.sumlikes = s->like[1];
+    r.sumsqlikes = s->like[2];
+    r.ancestdist = s->adist;
+    return r;
+}
+
+/* PRINTLIKES (structure.c:663-667, PrintLike output.c:187-200): the lnP(D) of
+ * the sweep just run; a collecting sweep has it from the device, a burn-in
+ * sweep does not accumulate it and it is computed on the host from Q and P. */
+static int print_like(sgpu_ctx *ctx, const shost_params *p, const shost_input *in,
+                      const sgpu_model *m, int rep)
+{
+    int rc;
+    if (rep + 1 > p->burnin && p->computeprob) {
+        double like[3];
+        if ((rc = sgpu_get_state(ctx, SGPU_LIKE, like, sizeof(like)))) {
+            return rc;
+        }
+        printf("%6.0f\n", like[0]);
+        return SGPU_OK;
+    }
+    const size_t nq = (size_t)m->N * m->K, np = (size_t)m->L * m->K * m->A;
+    double *Q = (double *)malloc(sizeof(double) * nq), *P = (double *)malloc(sizeof(double) * np);
+    rc = (Q && P) ? SGPU_OK : SGPU_ERR_NOMEM;
+    if (rc == SGPU_OK && (rc = sgpu_get_state(ctx, SGPU_Q, Q, sizeof(double) * nq)) == SGPU_OK &&
+        (rc = sgpu_get_state(ctx, SGPU_P, P, sizeof(double) * np)) == SGPU_OK) {
+        printf("%6.0f\n", shost_calc_like(in, m->K, Q, P));
+    }
+    free(Q);
+    free(P);
+    return rc;
 }
 
 /* One progress line (the columns of PrintUpdate, output.c:218-375). */
@@ -226,8 +296,24 @@ static int run_one(shost_params *p, shost_input *in, const sgpu_data *d, int A, 
     if (!s.alpha || !s.fst) {
         rc = SGPU_ERR_NOMEM;
     }
+    /* sweeps run in stretches that end where the reference reads the device: a
+     * progress line (UPDATEFREQ), an INTERMEDSAVE file, or every sweep with
+     * PRINTLIKES */
+    const int savefreq = shost_save_freq(p);
     while (rc == SGPU_OK && rep < total) {
         int n = p->updatefreq - rep % p->updatefreq;
+        if (p->printlikes) {
+            n = 1;
+        }
+        if (savefreq > 0) {
+            /* next sweep r (0-based) with (r + 1 - BURNIN) % savefreq == 0 */
+            int r = rep < p->burnin ? p->burnin : rep;
+            const int into = (r + 1 - p->burnin) % savefreq;
+            r += into ? savefreq - into : 0;
+            if (r + 1 - rep < n) {
+                n = r + 1 - rep;
+            }
+        }
         if (rep + n > total) {
             n = total - rep;
         }
@@ -235,6 +321,27 @@ static int run_one(shost_params *p, shost_input *in, const sgpu_data *d, int A, 
             break;
         }
         rep += n;
+        const int last = rep - 1;   /* the sweep just run */
+        if (savefreq && last + 1 > p->burnin && (last + 1 - p->burnin) % savefreq == 0 &&
+            last + 1 != total) {
+            /* an intermediate result file (structure.c:640-661) */
+            sums_t t;
+            if ((rc = read_sums(ctx, 0, &m, &t)) == SGPU_OK) {
+                shost_results r = results_of(rep, &t);
+                r.saveindex = (rep - p->burnin) / savefreq;
+                rc = shost_write_results(p, in, &r, 0, argc, argv);
+            }
+            free_sums(&t);
+            if (rc) {
+                break;
+            }
+        }
+        if (p->printlikes && (rc = print_like(ctx, p, in, &m, last))) {
+            break;
+        }
+        if (rep % p->updatefreq != 0 && rep != total) {
+            continue;
+        }
         /* the progress line needs alpha, F and the likelihood sums only */
         const size_t K = (size_t)m.K;
         if ((rc = sgpu_get_state(ctx, SGPU_ALPHA, s.alpha, sizeof(double) * K)) ||
@@ -252,8 +359,7 @@ static int run_one(shost_params *p, shost_input *in, const sgpu_data *d, int A, 
         rc = read_sums(ctx, 0, &m, &s);
     }
     if (rc == SGPU_OK) {
-        shost_results r = {rep, s.qsum, s.psum, s.epssum, s.fstsum, s.alphasum, s.lambdasum,
-                           s.like[1], s.like[2]};
+        shost_results r = results_of(rep, &s);
         rc = shost_write_results(p, in, &r, 1, argc, argv);
     }
     free_sums(&s);
@@ -329,8 +435,7 @@ static int run_ksweep(shost_params *p, shost_input *in, const sgpu_data *d, int 
                 strcpy(pk.outfile, saved);
                 strcat(pk.outfile, tag);
                 pk.seed = (int32_t)(seed + (uint32_t)r);
-                shost_results res = {total, s.qsum, s.psum, s.epssum, s.fstsum, s.alphasum,
-                                     s.lambdasum, s.like[1], s.like[2]};
+                shost_results res = results_of(total, &s);
                 rc = shost_write_results(&pk, in, &res, 2 /* <name>_f, no stdout */, argc, argv);
                 const int n = total - p->burnin;
                 for (size_t j = 0; j < nq; j++) {

@@ -57,6 +57,19 @@ def main():
         f.write(txt)
     with open(os.path.join(OUT, "output_v0_q.txt"), "w") as f:
         f.write(q)
+    # the ANCESTDIST variant: probability intervals in the ancestry table
+    variant = OUTPUT_VARIANTS[4]
+    K = variant["maxpops"]
+    st = result_state(200, 5, K, t["maxalleles"], t["nall"], seed=4,
+                      numboxes=variant["numboxes"])
+    pr.update(variant)
+    pr["maxpops"] = K
+    with tempfile.TemporaryDirectory() as d:
+        pr["outfile"] = os.path.join(d, "theirs")
+        ref.output_results(pr, t, 137, **st)
+        txt = open(pr["outfile"] + "_f").read().replace(pr["outfile"], "X")
+    with open(os.path.join(OUT, "output_v4_f.txt"), "w") as f:
+        f.write(txt)
     print("wrote", OUT)
 
 

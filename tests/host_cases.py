@@ -110,13 +110,21 @@ OUTPUT_VARIANTS = [
          popspecificlambda=1, printqhat=1, randomize=0, computeprob=1),
     dict(maxpops=1, popdata=1, freqscorr=1, onefst=0, popalphas=0, inferlambda=0,
          printqhat=0, computeprob=0),
+    # ANCESTDIST probability intervals (PrintAncestDist, output.c:762-797) and the
+    # PFROMPOPFLAGONLY line of PrintSums (output.c:1115-1117)
+    dict(maxpops=3, popdata=1, freqscorr=1, onefst=0, popalphas=0, inferlambda=0,
+         printqhat=1, ancestdist=1, numboxes=50, ancestpint=0.8),
+    dict(maxpops=4, popdata=0, freqscorr=0, onefst=0, popalphas=1, inferlambda=1,
+         printqhat=0, ancestdist=1, numboxes=1000, ancestpint=0.95, pfrompopflagonly=1),
 ]
 
 
-def result_state(N, L, K, A, nall, seed, nrep=37):
-    """Posterior sums of nrep collected sweeps with plausible values."""
+def result_state(N, L, K, A, nall, seed, nrep=37, numboxes=None):
+    """Posterior sums of nrep collected sweeps with plausible values; with
+    numboxes, also the ANCESTDIST histogram of the same Q draws."""
     rng = np.random.default_rng(seed)
-    q = rng.dirichlet(np.full(K, 0.7), size=(nrep, N)).sum(axis=0)
+    qs = rng.dirichlet(np.full(K, 0.7), size=(nrep, N))
+    q = qs.sum(axis=0)
     P = np.zeros((L, K, A))
     E = np.zeros((L, A))
     for l in range(L):
@@ -124,6 +132,15 @@ def result_state(N, L, K, A, nall, seed, nrep=37):
             P[l, :, :nall[l]] = rng.dirichlet(np.ones(nall[l]), size=(nrep, K)).sum(axis=0)
             E[l, :nall[l]] = rng.dirichlet(np.ones(nall[l]), size=nrep).sum(axis=0)
     likes = -1000.0 * (1 + rng.random(nrep))
-    return dict(qsum=q, psum=P, epssum=E, fstsum=rng.uniform(0, 0.3, K) * nrep,
-                alphasum=rng.uniform(0.2, 2, K) * nrep, lambdasum=rng.uniform(0.5, 2, K) * nrep,
-                sumlikes=float(likes.sum()), sumsqlikes=float((likes ** 2).sum()))
+    out = dict(qsum=q, psum=P, epssum=E, fstsum=rng.uniform(0, 0.3, K) * nrep,
+               alphasum=rng.uniform(0.2, 2, K) * nrep, lambdasum=rng.uniform(0.5, 2, K) * nrep,
+               sumlikes=float(likes.sum()), sumsqlikes=float((likes ** 2).sum()))
+    if numboxes:
+        box = np.minimum((qs * numboxes).astype(np.int64), numboxes - 1)
+        ad = np.zeros((N, K, numboxes), np.int32)
+        for r in range(nrep):
+            for i in range(N):
+                for k in range(K):
+                    ad[i, k, box[r, i, k]] += 1
+        out["ancestdist"] = ad
+    return out

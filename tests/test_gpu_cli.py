@@ -134,3 +134,69 @@ def test_ksweep_batch_equals_single_runs(tmp_path):
         return [ln for ln in t.splitlines()
                 if not ln.startswith(("Command line", "DATAFILE", "Output File"))]
     assert body(a) == body(b)
+
+
+OPTIONS = """#define ANCESTDIST 1
+#define NUMBOXES 100
+#define ANCESTPINT 0.9
+#define INTERMEDSAVE 2
+#define PRINTLIKES 1
+#define PFROMPOPFLAGONLY 1
+"""
+
+
+def test_structure_options_match_oracle_chain(tmp_path):
+    """ANCESTDIST (intervals in the ancestry table), INTERMEDSAVE (result files
+    <OUTFILE>_1, _2 at (rep - BURNIN) / savefreq, structure.c:640-661),
+    PRINTLIKES (one lnP(D) line per sweep, structure.c:663-667) and
+    PFROMPOPFLAGONLY, against the oracle chain with the same key."""
+    _files(tmp_path)
+    (tmp_path / "extraparams").write_text(EXTRA + OPTIONS)
+    out = _run(tmp_path)
+    p = host.get_params(["structure", "-m", str(tmp_path / "mainparams"), "-e",
+                         str(tmp_path / "extraparams")])
+    seed, K = 4242, 2
+    p.seed = seed
+    inp = host.InputFile(p)
+    assert inp.popflag.min() >= 0 and 0 < (inp.popflag == 1).sum() < inp.N
+    m = oracle.make_model(inp.N, inp.L, K, inp.A, burnin=p.burnin, numreps=p.numreps,
+                          freqscorr=p.freqscorr, inferalpha=p.inferalpha,
+                          inferlambda=p.inferlambda, metrofreq=p.metrofreq, alpha0=p.alpha,
+                          lambda0=p.lambda_, alphapropsd=p.alphapropsd,
+                          fpriormean=p.fpriormean, fpriorsd=p.fpriorsd, seed=seed, chain=K,
+                          ancestdist=1, numboxes=100, pfrompopflagonly=1)
+    ch = oracle.Chain(m, inp.geno_int8(), inp.nall, popflag=inp.popflag)
+    ch.init()
+    total = p.burnin + p.numreps
+    savefreq = host.save_freq(p)
+    assert savefreq == 20
+    likes = []
+    for rep in range(total):
+        ch.sweep(rep)
+        likes.append(ch.like if rep + 1 > p.burnin else ch.calc_like(ch.Q, ch.P))
+        n = rep + 1 - p.burnin
+        if n > 0 and n % savefreq == 0 and rep + 1 != total:
+            pi = host.get_params(["structure", "-m", str(tmp_path / "mainparams"), "-e",
+                                  str(tmp_path / "extraparams")])
+            pi.seed = seed
+            pi.outfile = str(tmp_path / "oracle").encode()
+            inp.write_results(rep + 1, ch.QSum, ch.PSum, ch.epsSum, ch.FstSum, ch.alphaSum,
+                              ch.lambdaSum, ch.sumlikes, ch.sumsqlikes, final=0, params=pi,
+                              ancestdist=ch.ancestDist, saveindex=n // savefreq)
+    pi.outfile = str(tmp_path / "oracle").encode()
+    inp.write_results(total, ch.QSum, ch.PSum, ch.epsSum, ch.FstSum, ch.alphaSum, ch.lambdaSum,
+                      ch.sumlikes, ch.sumsqlikes, final=2, params=pi, ancestdist=ch.ancestDist)
+    printed = [int(x) for x in re.findall(r"^ *(-?\d+)$", out, flags=re.M)]
+    assert len(printed) == total
+    assert np.all(np.abs(np.array(printed) - np.round(likes)) <= 1), "PRINTLIKES lines"
+    for ours, theirs in [("run_1", "oracle_1"), ("run_2", "oracle_2"), ("run_f", "oracle_f")]:
+        a = (tmp_path / ours).read_text().replace(ours[:3], "X").splitlines()
+        b = (tmp_path / theirs).read_text().replace(str(tmp_path / "oracle"), "X").splitlines()
+        assert len(a) == len(b), ours
+        assert any("probability intervals" in ln for ln in a)
+        assert any("POPFLAG=1 ONLY" in ln for ln in a)
+        for la, lb in zip(a, b):
+            if la != lb:   # FP64 trajectories agree to ~1e-9: printed digits may differ
+                na, nb = _numbers(la), _numbers(lb)
+                assert len(na) == len(nb) and np.allclose(na, nb, rtol=0, atol=0.11), (la, lb)
+    assert not (tmp_path / "run_3").exists()

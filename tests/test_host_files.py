@@ -168,7 +168,8 @@ def test_output_matches_reference(ref, vi, tmp_path):
     variant = OUTPUT_VARIANTS[vi]
     K = variant["maxpops"]
     inp = _read_ours(dict(TESTDATA1_PARAMS, datafile=TESTDATA1))
-    st = result_state(inp.N, inp.L, K, inp.A, inp.nall, seed=vi)
+    st = result_state(inp.N, inp.L, K, inp.A, inp.nall, seed=vi,
+                      numboxes=variant.get("numboxes") if variant.get("ancestdist") else None)
     p = _output_params(variant, K)
     p.outfile = str(tmp_path / "ours").encode()
     inp.write_results(137, **st, params=p)
@@ -181,6 +182,52 @@ def test_output_matches_reference(ref, vi, tmp_path):
     assert a.replace(str(tmp_path / "ours"), "X") == b.replace(str(tmp_path / "theirs"), "X")
     if variant.get("printqhat"):
         assert (tmp_path / "ours_q").read_text() == (tmp_path / "theirs_q").read_text()
+
+
+@pytest.mark.parametrize("intermedsave,numreps", [(3, 37), (1, 36), (4, 40)])
+def test_intermediate_files_match_reference(ref, intermedsave, numreps, tmp_path):
+    """INTERMEDSAVE (structure.c:640-661): the save interval NUMREPS /
+    (INTERMEDSAVE + 1) rounded up (Init.c:377-385), and each intermediate file
+    <OUTFILE>_<(rep - BURNIN) / savefreq> byte for byte as OutPutResults(final=0)
+    writes it, ANCESTDIST intervals included."""
+    from oracle import ref as R
+    variant = dict(OUTPUT_VARIANTS[4], numreps=numreps, intermedsave=intermedsave)
+    K = variant["maxpops"]
+    inp = _read_ours(dict(TESTDATA1_PARAMS, datafile=TESTDATA1))
+    p = _output_params(variant, K)
+    savefreq = host.save_freq(p)
+    expect = numreps // (intermedsave + 1)
+    expect += 1 if expect * (intermedsave + 1) < numreps else 0
+    assert savefreq == expect
+    theirs_in = R.read_input(dict(TESTDATA1_PARAMS, datafile=TESTDATA1))
+    for idx in range(1, intermedsave + 1):
+        rep = p.burnin + idx * savefreq
+        if rep >= p.burnin + numreps:
+            break
+        st = result_state(inp.N, inp.L, K, inp.A, inp.nall, seed=idx, nrep=rep - p.burnin,
+                          numboxes=variant["numboxes"])
+        p.outfile = str(tmp_path / "ours").encode()
+        inp.write_results(rep, **st, params=p, final=0, saveindex=(rep - p.burnin) // savefreq)
+        pd = p.as_dict()
+        pd["outfile"] = str(tmp_path / "theirs")
+        R.output_results(pd, theirs_in, rep, **st, final=0, savefreq=savefreq)
+        a = (tmp_path / f"ours_{idx}").read_text()
+        b = (tmp_path / f"theirs_{idx}").read_text()
+        assert a.replace(str(tmp_path / "ours"), "X") == b.replace(str(tmp_path / "theirs"), "X")
+
+
+def test_host_calc_like_equals_oracle():
+    """PRINTLIKES's burn-in lnP(D) on the host equals the oracle's CalcLike."""
+    import oracle
+    inp = _read_ours(dict(TESTDATA1_PARAMS, datafile=TESTDATA1))
+    rng = np.random.default_rng(2)
+    K = 3
+    Q = rng.dirichlet(np.ones(K), size=inp.N)
+    P = np.zeros((inp.L, K, inp.A))
+    for l in range(inp.L):
+        P[l, :, :inp.nall[l]] = rng.dirichlet(np.ones(inp.nall[l]), size=K)
+    ch = oracle.Chain(oracle.make_model(inp.N, inp.L, K, inp.A), inp.geno_int8(), inp.nall)
+    assert inp.calc_like(Q, P) == ch.calc_like(Q, P)
 
 
 def test_output_golden(tmp_path):
@@ -196,6 +243,20 @@ def test_output_golden(tmp_path):
         assert got == f.read()
     with open(os.path.join(GOLD, "output_v0_q.txt")) as f:
         assert (tmp_path / "ours_q").read_text() == f.read()
+
+
+def test_output_golden_ancestdist(tmp_path):
+    """The ANCESTDIST variant against the reference's file kept as a fixture."""
+    variant = OUTPUT_VARIANTS[4]
+    K = variant["maxpops"]
+    inp = _read_ours(dict(TESTDATA1_PARAMS, datafile=TESTDATA1))
+    st = result_state(inp.N, inp.L, K, inp.A, inp.nall, seed=4, numboxes=variant["numboxes"])
+    p = _output_params(variant, K)
+    p.outfile = str(tmp_path / "ours").encode()
+    inp.write_results(137, **st, params=p, final=2)
+    got = (tmp_path / "ours_f").read_text().replace(str(tmp_path / "ours"), "X")
+    with open(os.path.join(GOLD, "output_v4_f.txt")) as f:
+        assert got == f.read()
 
 
 def test_est_log_prob_closed_form():
