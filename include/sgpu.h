@@ -249,6 +249,19 @@ int sgpu_synchronize(sgpu_ctx *ctx);
 enum { SGPU_FN_LOG = 0, SGPU_FN_COS = 1, SGPU_FN_LGAMMA = 2, SGPU_FN_SIN = 3 };
 int sgpu_eval_math(int32_t fn, const double *x, double *y, int64_t n);
 
+/* sgpu_eval_math without any device allocation: no counterpart in the
+ * reference, for callers that cannot allocate device memory (harnesses that
+ * forbid hipMalloc, allocation-free serving loops).  x and y are host arrays
+ * of n <= SGPU_EVAL_MATH_NOALLOC_MAX values; larger n returns SGPU_ERR_ARG.
+ * The values are staged in a fixed device array of the library (1 MiB per GPU,
+ * held until the process ends) that every call on that GPU shares: calls that
+ * overlapped on one GPU would race, so the library runs them one at a time (a
+ * process-wide lock around each synchronous call).  Nothing in the staging
+ * needs clearing: every value a call reads was written by that call.  Runs on
+ * the current HIP device. */
+#define SGPU_EVAL_MATH_NOALLOC_MAX 65536
+int sgpu_eval_math_noalloc(int32_t fn, const double *x, double *y, int64_t n);
+
 /* Message for the last failing call on this thread. */
 const char *sgpu_last_error(void);
 

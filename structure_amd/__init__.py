@@ -110,6 +110,7 @@ def lib() -> C.CDLL:
             "sgpu_set_timing": (C.c_int, [vp, i32]),
             "sgpu_get_timing": (C.c_int, [vp, vp, vp]),
             "sgpu_eval_math": (C.c_int, [i32, vp, vp, C.c_int64]),
+            "sgpu_eval_math_noalloc": (C.c_int, [i32, vp, vp, C.c_int64]),
             "sgpu_last_error": (C.c_char_p, []),
             "sgpu_version": (C.c_char_p, []),
             "sgpu_synth_snp": (C.c_int, [i32, i32, i32, C.c_double, C.c_double, C.c_double,
@@ -139,11 +140,17 @@ def _vp(a: np.ndarray):
 FN_LOG, FN_COS, FN_LGAMMA, FN_SIN = 0, 1, 2, 3
 
 
-def eval_math(fn: int, x) -> np.ndarray:
-    """The device's own log / cos / sin / lgamma over an array (diagnostics)."""
+EVAL_MATH_NOALLOC_MAX = 65536   # SGPU_EVAL_MATH_NOALLOC_MAX
+
+
+def eval_math(fn: int, x, noalloc: bool = False) -> np.ndarray:
+    """The device's own log / cos / sin / lgamma over an array (diagnostics);
+    noalloc: through the allocation-free entry point (at most
+    EVAL_MATH_NOALLOC_MAX values)."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.empty_like(x)
-    _check(lib().sgpu_eval_math(fn, _vp(x), _vp(y), x.size))
+    f = lib().sgpu_eval_math_noalloc if noalloc else lib().sgpu_eval_math
+    _check(f(fn, _vp(x), _vp(y), x.size))
     return y
 
 

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -492,6 +493,20 @@ __global__ void k_eval_math(int fn, const double *__restrict__ x, double *__rest
     y[i] = fn == SGPU_FN_LOG ? log_d(v)
                              : (fn == SGPU_FN_COS ? cs : (fn == SGPU_FN_SIN ? sn : lgamma_lanczos(v)));
 }
+
+}  // namespace
+
+// Device staging of sgpu_eval_math_noalloc (x, then y), allocated with the code
+// object: each GPU holds its own copy until the process ends.  Host code never
+// uses this name as an address, only the pointer hipGetSymbolAddress returns
+// for the current device (g_stage_ptr).
+static __device__ __align__(256) double scratch[2 * SGPU_EVAL_MATH_NOALLOC_MAX];
+
+namespace {
+
+constexpr int kStageDevices = 64;
+std::mutex g_stage_mu;
+double *g_stage_ptr[kStageDevices] = {};   // address of `scratch` on each device
 
 struct DevBuf {
     void *p = nullptr;
@@ -1663,6 +1678,47 @@ int sgpu_eval_math(int32_t fn, const double *x, double *y, int64_t n)
                        (const double *)dx.p, (double *)dy.p, n);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpy(y, dy.p, bytes, hipMemcpyDeviceToHost));
+    return SGPU_OK;
+}
+
+int sgpu_eval_math_noalloc(int32_t fn, const double *x, double *y, int64_t n)
+{
+    if (fn < SGPU_FN_LOG || fn > SGPU_FN_SIN || n < 0 || (n > 0 && (!x || !y))) {
+        return fail(SGPU_ERR_ARG, "sgpu_eval_math_noalloc: fn %d, n %lld", fn, (long long)n);
+    }
+    if (n > SGPU_EVAL_MATH_NOALLOC_MAX) {
+        return fail(SGPU_ERR_ARG, "sgpu_eval_math_noalloc: n = %lld exceeds the %d values of its "
+                                  "static staging", (long long)n, SGPU_EVAL_MATH_NOALLOC_MAX);
+    }
+    if (n == 0) {
+        return SGPU_OK;
+    }
+    // one call at a time per process: the staging is shared by every call on a GPU
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kStageDevices) {
+        return fail(SGPU_ERR_DEVICE, "sgpu_eval_math_noalloc: device %d beyond %d", dev,
+                    kStageDevices);
+    }
+    if (!g_stage_ptr[dev]) {
+        // once per device, that device current: the array's address on it
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            return fail(SGPU_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s",
+                        hipGetErrorString(e));
+        }
+        g_stage_ptr[dev] = (double *)p;
+    }
+    double *sx = g_stage_ptr[dev];                  // x staged in the first half
+    double *sy = sx + SGPU_EVAL_MATH_NOALLOC_MAX;   // y in the second
+    const size_t bytes = (size_t)n * sizeof(double);
+    HIP_OK(hipMemcpy(sx, x, bytes, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_eval_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, fn,
+                       (const double *)sx, sy, n);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpy(y, sy, bytes, hipMemcpyDeviceToHost));
     return SGPU_OK;
 }
 
