@@ -40,7 +40,7 @@ enum {
      * uint32 laid out [L][Gpad], locus-major: word (h, l) at l * Gpad + h holds
      * bytes (ind 2h copy 0, ind 2h copy 1, ind 2h+1 copy 0, ind 2h+1 copy 1), low
      * byte first.  Gpad = ceil(N/2) rounded up to 64; padding words are all 255.
-     * Allele indices are < 127 (int8 input). */
+     * Allele indices are < 255 (MAXALLELES <= 255; sgpu_pack_byte32). */
     SGPU_GENO_BYTE = 2
 };
 
@@ -119,6 +119,12 @@ size_t sgpu_packed_words(int32_t fmt, int32_t N, int32_t L);
  * as (1,0)); such data goes through SGPU_GENO_BYTE, which keeps file order. */
 int sgpu_pack_snp2(const int8_t *geno, int32_t N, int32_t L, uint32_t *out, size_t out_words);
 int sgpu_pack_byte(const int8_t *geno, int32_t N, int32_t L, uint32_t *out, size_t out_words);
+
+/* SGPU_GENO_BYTE from the reference's own Geno array (int32 [N][2][L], GenPos,
+ * structure.h:22) with its missing code: alleles 0..254, so loci with up to
+ * 255 alleles (the int8 form above stops at 127). */
+int sgpu_pack_byte32(const int32_t *geno, int32_t N, int32_t L, int32_t missing, uint32_t *out,
+                     size_t out_words);
 
 /* ---- context ------------------------------------------------------------ */
 

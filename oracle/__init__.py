@@ -67,7 +67,7 @@ def make_model(N, L, K, A, **kw) -> Model:
 
 _D = C.POINTER(C.c_double)
 _I = C.POINTER(C.c_int32)
-_B = C.POINTER(C.c_int8)
+_B = C.POINTER(C.c_int16)   # genotypes [N][2][L] int16 (-9 missing)
 _U = C.POINTER(C.c_uint32)
 _M = C.POINTER(Model)
 
@@ -140,8 +140,8 @@ def ip(a):
 
 
 def bp(a):
-    assert a.dtype == np.int8
-    return ptr(a, C.c_int8)
+    assert a.dtype == np.int16
+    return ptr(a, C.c_int16)
 
 
 # ---- scalar helpers ---------------------------------------------------------
@@ -207,7 +207,8 @@ def snp2_counts(words: np.ndarray, N: int, L: int, threads=0):
 
 @dataclass
 class Chain:
-    """One oracle chain: model, genotypes ([N][2][L] int8, -9 missing), state;
+    """One oracle chain: model, genotypes ([N][2][L], -9 missing; int16 inside,
+    so alleles up to 254 fit), state;
     popflag ([N], the data file's POPFLAG column) for PFROMPOPFLAGONLY."""
 
     model: Model
@@ -219,7 +220,7 @@ class Chain:
     def __post_init__(self):
         m = self.model
         N, L, K, A = m.N, m.L, m.K, m.A
-        self.geno = np.ascontiguousarray(self.geno, dtype=np.int8)
+        self.geno = np.ascontiguousarray(self.geno, dtype=np.int16)
         assert self.geno.shape == (N, 2, L)
         self.nall = np.ascontiguousarray(self.nall, dtype=np.int32)
         if self.popflag is not None:

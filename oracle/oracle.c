@@ -156,9 +156,9 @@ static int orc_counts_alleles(const orc_model *m, int ind)
 /* ------------------------------------------------------------------------
  * Counts from an explicit Z: GetNumFromPops (UpdateP.c:60-99, with its
  * PFROMPOPFLAGONLY filter) and GetNumLociPop (UpdateQ.c:17-35).
- * geno is [N][2][L] int8 with ORC_MISSING for missing copies.
+ * geno is [N][2][L] int16 with ORC_MISSING for missing copies.
  * ---------------------------------------------------------------------- */
-void orc_count_from_z(const orc_model *m, const int8_t *geno, const int32_t *Z,
+void orc_count_from_z(const orc_model *m, const int16_t *geno, const int32_t *Z,
                       int32_t *nA, int32_t *nQ)
 {
     const int N = m->N, L = m->L, K = m->K, A = m->A;
@@ -185,7 +185,7 @@ void orc_count_from_z(const orc_model *m, const int8_t *geno, const int32_t *Z,
  * numToRange (ran.c:1083-1087): Cutoffs[k] = Q[i,k]*P[l,k,x], summed in pop
  * order, r = 0 + u*(sum-0), first k with r <= running sum.  The uniform for
  * (i, l, copy) comes from phase 1.  Counts are accumulated directly. */
-void orc_update_z(const orc_model *m, const int8_t *geno, const double *Q, const double *P,
+void orc_update_z(const orc_model *m, const int16_t *geno, const double *Q, const double *P,
                   uint32_t rep, int32_t *Z, int32_t *nA, int32_t *nQ)
 {
     const int N = m->N, L = m->L, K = m->K, A = m->A;
@@ -236,7 +236,7 @@ void orc_update_z(const orc_model *m, const int8_t *geno, const double *Q, const
 
 /* InitializeZ (Init.c:124-169): every non-missing copy gets
  * RandomInteger(0, K-1) = (int)(u*K) (ran.c:138-147), u from phase 2. */
-void orc_init_z(const orc_model *m, const int8_t *geno, int32_t *Z, int32_t *nA, int32_t *nQ)
+void orc_init_z(const orc_model *m, const int16_t *geno, int32_t *Z, int32_t *nA, int32_t *nQ)
 {
     const int N = m->N, L = m->L, K = m->K, A = m->A;
     memset(nA, 0, sizeof(int32_t) * (size_t)L * K * A);
@@ -326,7 +326,7 @@ void orc_update_q_gibbs(const orc_model *m, const int32_t *nQ, const double *alp
 
 /* CalcLikeIndDiff (CalcLike.c:281-345): sum over copies of
  * log(Qplus.P) - log(Qminus.P), line-major. */
-double orc_calc_like_ind_diff(const orc_model *m, const int8_t *geno, const double *Qplus,
+double orc_calc_like_ind_diff(const orc_model *m, const int16_t *geno, const double *Qplus,
                               const double *Qminus, const double *P, int ind)
 {
     const int L = m->L, K = m->K, A = m->A;
@@ -351,7 +351,7 @@ double orc_calc_like_ind_diff(const orc_model *m, const int8_t *geno, const doub
 
 /* UpdateQMetro (UpdateQ.c:199-307): proposal TestQ ~ Dirichlet(alpha)
  * (RDirichletDisc, ran.c:1198-1210), accepted if u < exp(logdiff). */
-void orc_update_q_metro(const orc_model *m, const int8_t *geno, const double *P,
+void orc_update_q_metro(const orc_model *m, const int16_t *geno, const double *P,
                         const double *alpha, uint32_t rep, double *Q, int32_t *accepted)
 {
     const int N = m->N, K = m->K;
@@ -694,7 +694,7 @@ void orc_update_fst(const orc_model *m, const int32_t *nall, const double *eps, 
 /* CalcLike (CalcLike.c:387-426) -> CalcLikeInd (:75-133): per individual a
  * clamped running product of Q.P over (line, locus), logged whenever it
  * drops below sqrt(UNDERFLO). */
-double orc_calc_like(const orc_model *m, const int8_t *geno, const double *Q, const double *P)
+double orc_calc_like(const orc_model *m, const int16_t *geno, const double *Q, const double *P)
 {
     const int N = m->N, L = m->L, K = m->K, A = m->A;
     double loglike = 0.0;
@@ -725,7 +725,7 @@ double orc_calc_like(const orc_model *m, const int8_t *geno, const double *Q, co
 /* DataCollection (output.c:50-101) -> UpdateSums (output.c:654-707):
  * posterior sums, the ANCESTDIST histogram (output.c:691-700) and the lnP(D)
  * accumulators (LOCPRIOR=0). */
-void orc_collect(const orc_model *m, const int8_t *geno, const int32_t *nall, orc_state *s)
+void orc_collect(const orc_model *m, const int16_t *geno, const int32_t *nall, orc_state *s)
 {
     const int N = m->N, L = m->L, K = m->K, A = m->A;
     for (int k = 0; k < K; k++) {
@@ -778,7 +778,7 @@ void orc_collect(const orc_model *m, const int8_t *geno, const int32_t *nall, or
 /* Initialization (Init.c:329-444) for the admixture model: alpha, lambda,
  * InitFreqPriors (Init.c:172-240), Q = 1/K (structure.c:51-58), random Z
  * (Init.c:124-169) reduced to its counts, zeroed sums. */
-void orc_init_state(const orc_model *m, const int8_t *geno, const int32_t *nall, orc_state *s)
+void orc_init_state(const orc_model *m, const int16_t *geno, const int32_t *nall, orc_state *s)
 {
     const int N = m->N, L = m->L, K = m->K, A = m->A;
     for (int k = 0; k < K; k++) {
@@ -844,7 +844,7 @@ void orc_init_state(const orc_model *m, const int8_t *geno, const int32_t *nall,
 /* One sweep in the order of the reference hot loop (structure.c:497-638):
  * P, Q (Metropolis when rep % METROFREQ == 0, UpdateQ.c:740), Z, alpha,
  * lambda, epsilon + Fst, then data collection once rep + 1 > BURNIN. */
-void orc_sweep(const orc_model *m, const int8_t *geno, const int32_t *nall, uint32_t rep,
+void orc_sweep(const orc_model *m, const int16_t *geno, const int32_t *nall, uint32_t rep,
                orc_state *s)
 {
     orc_update_p(m, nall, s->nA, s->eps, s->F, s->lambda, rep, s->P);

@@ -63,7 +63,8 @@ typedef struct {
     const int32_t *popflag;     /* [N] Individual[i].PopFlag, or NULL */
 } orc_model;
 
-/* Chain state.  Layouts follow structure.h:21-33 (GenPos, PPos, QPos, EpsPos). */
+/* Chain state.  Layouts follow structure.h:21-33 (GenPos, PPos, QPos, EpsPos);
+ * genotypes are int16 [N][2][L] (alleles 0..254, ORC_MISSING for a missing copy). */
 typedef struct {
     double *Q;        /* [N][K]   */
     double *P;        /* [L][K][A] */
@@ -92,21 +93,21 @@ double orc_gamma(double a, uint32_t seed, uint32_t chain, uint32_t c0, uint32_t 
 double orc_lgamma(double z);
 
 /* --- initialisation (Init.c:124-240, 350-414) --------------------------- */
-void orc_init_state(const orc_model *m, const int8_t *geno, const int32_t *nall, orc_state *s);
+void orc_init_state(const orc_model *m, const int16_t *geno, const int32_t *nall, orc_state *s);
 
 /* --- per-phase updates --------------------------------------------------- */
-void orc_count_from_z(const orc_model *m, const int8_t *geno, const int32_t *Z,
+void orc_count_from_z(const orc_model *m, const int16_t *geno, const int32_t *Z,
                       int32_t *nA, int32_t *nQ);
-void orc_update_z(const orc_model *m, const int8_t *geno, const double *Q, const double *P,
+void orc_update_z(const orc_model *m, const int16_t *geno, const double *Q, const double *P,
                   uint32_t rep, int32_t *Z /* may be NULL */, int32_t *nA, int32_t *nQ);
-void orc_init_z(const orc_model *m, const int8_t *geno, int32_t *Z, int32_t *nA, int32_t *nQ);
+void orc_init_z(const orc_model *m, const int16_t *geno, int32_t *Z, int32_t *nA, int32_t *nQ);
 void orc_update_p(const orc_model *m, const int32_t *nall, const int32_t *nA, const double *eps,
                   const double *F, const double *lambda, uint32_t rep, double *P);
 void orc_update_q_gibbs(const orc_model *m, const int32_t *nQ, const double *alpha,
                         uint32_t rep, double *Q);
-double orc_calc_like_ind_diff(const orc_model *m, const int8_t *geno, const double *Qplus,
+double orc_calc_like_ind_diff(const orc_model *m, const int16_t *geno, const double *Qplus,
                               const double *Qminus, const double *P, int ind);
-void orc_update_q_metro(const orc_model *m, const int8_t *geno, const double *P,
+void orc_update_q_metro(const orc_model *m, const int16_t *geno, const double *P,
                         const double *alpha, uint32_t rep, double *Q, int32_t *accepted);
 double orc_log_prob_q(const orc_model *m, const double *Q, double onealpha);
 double orc_log_prob_q_onepop(const orc_model *m, const double *Q, double alpha,
@@ -133,9 +134,9 @@ void orc_update_fst_given(const orc_model *m, const int32_t *nall, const double 
                           const double *P, double *F, const double *z, const double *u);
 void orc_update_fst(const orc_model *m, const int32_t *nall, const double *eps, const double *P,
                     uint32_t rep, double *F);
-double orc_calc_like(const orc_model *m, const int8_t *geno, const double *Q, const double *P);
-void orc_collect(const orc_model *m, const int8_t *geno, const int32_t *nall, orc_state *s);
-void orc_sweep(const orc_model *m, const int8_t *geno, const int32_t *nall, uint32_t rep,
+double orc_calc_like(const orc_model *m, const int16_t *geno, const double *Q, const double *P);
+void orc_collect(const orc_model *m, const int16_t *geno, const int32_t *nall, orc_state *s);
+void orc_sweep(const orc_model *m, const int16_t *geno, const int32_t *nall, uint32_t rep,
                orc_state *s);
 
 /* --- 2-bit genotype words, host threads (oracle_packed.c) -----------------

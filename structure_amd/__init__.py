@@ -86,6 +86,7 @@ def lib() -> C.CDLL:
             "sgpu_packed_words": (sz, [i32, i32, i32]),
             "sgpu_pack_snp2": (C.c_int, [vp, i32, i32, vp, sz]),
             "sgpu_pack_byte": (C.c_int, [vp, i32, i32, vp, sz]),
+            "sgpu_pack_byte32": (C.c_int, [vp, i32, i32, i32, vp, sz]),
             "sgpu_create": (C.c_int, [C.POINTER(Model), C.POINTER(Data), C.POINTER(vp)]),
             "sgpu_create_batch": (C.c_int, [C.POINTER(Model), i32, C.POINTER(Data), i32, vp,
                                             C.POINTER(Tuning), C.POINTER(vp)]),
@@ -161,16 +162,27 @@ def packed_words(fmt, N, L) -> int:
 
 
 def pack(geno: np.ndarray, fmt: int | None = None):
-    """Pack recoded genotypes [N][2][L] (int8, -9 missing).  fmt=None picks SNP2
-    when every site is representable, else the byte format.  Returns (fmt, words)."""
-    geno = np.ascontiguousarray(geno, dtype=np.int8)
+    """Pack recoded genotypes [N][2][L] (-9 missing; alleles up to 254 in any
+    integer type).  fmt=None picks SNP2 when every site is representable, else
+    the byte format.  Returns (fmt, words)."""
+    geno = np.asarray(geno)
     N, two, L = geno.shape
     assert two == 2
+    wide = geno.dtype != np.int8 and geno.size > 0 and int(geno.max()) > 126
+    g8 = None if wide else np.ascontiguousarray(geno, dtype=np.int8)
     order = [GENO_SNP2, GENO_BYTE] if fmt is None else [fmt]
+    if wide:
+        order = [f for f in order if f == GENO_BYTE] or order
     for f in order:
         out = np.empty(packed_words(f, N, L), np.uint32)
-        fn = lib().sgpu_pack_snp2 if f == GENO_SNP2 else lib().sgpu_pack_byte
-        rc = fn(_vp(geno), N, L, _vp(out), out.size)
+        if wide and f == GENO_BYTE:
+            g32 = np.ascontiguousarray(geno, dtype=np.int32)
+            rc = lib().sgpu_pack_byte32(_vp(g32), N, L, -9, _vp(out), out.size)
+        elif wide:
+            rc = ERR_UNSUPPORTED   # alleles above 1 are not 2-bit data
+        else:
+            fn = lib().sgpu_pack_snp2 if f == GENO_SNP2 else lib().sgpu_pack_byte
+            rc = fn(_vp(g8), N, L, _vp(out), out.size)
         if rc == OK:
             return f, out
         if fmt is not None or rc != ERR_UNSUPPORTED:

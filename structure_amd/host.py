@@ -184,6 +184,12 @@ class InputFile:
             raise SgpuError(rc, "alleles do not fit the packed formats")
         return out
 
+    def geno16(self) -> np.ndarray:
+        """Recoded genotypes [N][2][L] as int16 with -9 for missing (any allele count)."""
+        g = self.geno.astype(np.int16)
+        g[self.geno == self.missing] = -9
+        return g
+
     def write_results(self, rep, qsum, psum, epssum, fstsum, alphasum, lambdasum, sumlikes,
                       sumsqlikes, final=1, args=("structure",), params: Params | None = None,
                       ancestdist=None, saveindex=0):

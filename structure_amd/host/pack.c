@@ -71,37 +71,50 @@ int sgpu_pack_snp2(const int8_t *geno, int32_t N, int32_t L, uint32_t *out, size
     return SGPU_OK;
 }
 
+/* Byte format from any integer genotype type: G is the element type, miss the
+ * missing code.  Alleles 0..254; 255 marks a missing copy. */
+#define PACK_BYTE_BODY(G)                                                                  \
+    const size_t gp = gpad_of(N);                                                          \
+    if (!geno || !out || out_words != sgpu_packed_words(SGPU_GENO_BYTE, N, L)) {          \
+        return SGPU_ERR_ARG;                                                               \
+    }                                                                                      \
+    memset(out, 0xff, out_words * sizeof(uint32_t)); /* padding rows: copies missing */    \
+    /* loci in blocks of 64, so that both the reads along a copy and the writes       */  \
+    /* down a locus column stay within a few KiB                                       */  \
+    for (int32_t l0 = 0; l0 < L; l0 += 64) {                                               \
+        const int32_t l1 = l0 + 64 < L ? l0 + 64 : L;                                      \
+        for (int32_t i = 0; i < N; i++) {                                                  \
+            const int sh = 16 * (i & 1);                                                   \
+            for (int c = 0; c < 2; c++) {                                                  \
+                const G *g = geno + ((size_t)i * 2 + c) * L;                               \
+                const int s = sh + 8 * c;                                                  \
+                for (int32_t l = l0; l < l1; l++) {                                        \
+                    const int32_t x = (int32_t)g[l];                                       \
+                    uint32_t v;                                                            \
+                    if (x == miss) {                                                       \
+                        v = 0xffu;                                                         \
+                    } else if (x >= 0 && x < 255) {                                        \
+                        v = (uint32_t)x;                                                   \
+                    } else {                                                               \
+                        return SGPU_ERR_ARG;                                               \
+                    }                                                                      \
+                    uint32_t *w = out + (size_t)l * gp + (size_t)(i >> 1);                 \
+                    *w = (*w & ~(0xffu << s)) | (v << s);                                  \
+                }                                                                          \
+            }                                                                              \
+        }                                                                                  \
+    }                                                                                      \
+    return SGPU_OK;
+
 int sgpu_pack_byte(const int8_t *geno, int32_t N, int32_t L, uint32_t *out, size_t out_words)
 {
-    const size_t gp = gpad_of(N);
-    if (!geno || !out || out_words != sgpu_packed_words(SGPU_GENO_BYTE, N, L)) {
-        return SGPU_ERR_ARG;
-    }
-    memset(out, 0xff, out_words * sizeof(uint32_t));   /* padding rows: all copies missing */
-    /* loci in blocks of 64, so that both the reads along a copy and the writes
-     * down a locus column stay within a few KiB */
-    for (int32_t l0 = 0; l0 < L; l0 += 64) {
-        const int32_t l1 = l0 + 64 < L ? l0 + 64 : L;
-        for (int32_t i = 0; i < N; i++) {
-            const int sh = 16 * (i & 1);
-            for (int c = 0; c < 2; c++) {
-                const int8_t *g = geno + ((size_t)i * 2 + c) * L;
-                const int s = sh + 8 * c;
-                for (int32_t l = l0; l < l1; l++) {
-                    const int x = g[l];
-                    uint32_t v;
-                    if (x == -9) {
-                        v = 0xffu;
-                    } else if (x >= 0 && x < 255) {
-                        v = (uint32_t)x;
-                    } else {
-                        return SGPU_ERR_ARG;
-                    }
-                    uint32_t *w = out + (size_t)l * gp + (size_t)(i >> 1);
-                    *w = (*w & ~(0xffu << s)) | (v << s);
-                }
-            }
-        }
-    }
-    return SGPU_OK;
+    const int32_t miss = -9;
+    PACK_BYTE_BODY(int8_t)
+}
+
+int sgpu_pack_byte32(const int32_t *geno, int32_t N, int32_t L, int32_t missing, uint32_t *out,
+                     size_t out_words)
+{
+    const int32_t miss = missing;
+    PACK_BYTE_BODY(int32_t)
 }

@@ -92,37 +92,42 @@ static sgpu_model model_of(const shost_params *p, int K, int A, uint32_t seed, u
     return m;
 }
 
-/* Packed genotypes: 2-bit when every site is representable, else bytes. */
+/* Packed genotypes: 2-bit when every site is representable, else bytes
+ * straight from the recoded int32 genotypes (up to 255 alleles per locus). */
 static int pack(const shost_input *in, sgpu_data *d, uint32_t **words, int *A)
 {
-    const size_t n = (size_t)in->N * 2 * in->L;
-    int8_t *g8 = (int8_t *)malloc(n);
-    if (!g8) {
-        return SGPU_ERR_NOMEM;
-    }
-    int rc = shost_geno_int8(in, g8);
-    if (rc) {
-        free(g8);
-        return rc;
-    }
+    int rc = SGPU_OK;
     *A = in->maxalleles > 0 ? in->maxalleles : 1;
     int fmt = SGPU_GENO_BYTE;
     size_t nw = 0;
     *words = NULL;
     if (*A == 2) {
+        const size_t n = (size_t)in->N * 2 * in->L;
+        int8_t *g8 = (int8_t *)malloc(n);
+        if (!g8) {
+            return SGPU_ERR_NOMEM;
+        }
         nw = sgpu_packed_words(SGPU_GENO_SNP2, in->N, in->L);
         *words = (uint32_t *)malloc(nw * sizeof(uint32_t));
-        if (*words && sgpu_pack_snp2(g8, in->N, in->L, *words, nw) == SGPU_OK) {
+        if (*words && shost_geno_int8(in, g8) == SGPU_OK &&
+            sgpu_pack_snp2(g8, in->N, in->L, *words, nw) == SGPU_OK) {
             fmt = SGPU_GENO_SNP2;
         }
+        free(g8);
     }
     if (fmt == SGPU_GENO_BYTE) {
         free(*words);
+        if (in->lines != 2) {
+            return SGPU_ERR_UNSUPPORTED;
+        }
         nw = sgpu_packed_words(SGPU_GENO_BYTE, in->N, in->L);
         *words = (uint32_t *)malloc(nw * sizeof(uint32_t));
-        rc = *words ? sgpu_pack_byte(g8, in->N, in->L, *words, nw) : SGPU_ERR_NOMEM;
+        rc = *words ? sgpu_pack_byte32(in->geno, in->N, in->L, in->missing, *words, nw)
+                    : SGPU_ERR_NOMEM;
+        if (rc == SGPU_ERR_ARG) {
+            printf("allele codes do not fit the byte genotype format (at most 255 alleles)\n");
+        }
     }
-    free(g8);
     d->format = fmt;
     d->geno = *words;
     d->geno_words = nw;

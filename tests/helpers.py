@@ -17,7 +17,7 @@ def random_geno(rng, N, L, A_max, miss=0.05, biallelic=False):
         nall = np.full(L, 2, np.int32)
     else:
         nall = rng.integers(1, A_max + 1, size=L).astype(np.int32)
-    geno = np.empty((N, 2, L), np.int8)
+    geno = np.empty((N, 2, L), np.int8 if A_max <= 127 else np.int16)
     for l in range(L):
         geno[:, :, l] = rng.integers(0, nall[l], size=(N, 2))
     geno[rng.random((N, 2, L)) < miss] = -9

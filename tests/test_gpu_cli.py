@@ -85,7 +85,7 @@ def _oracle_file(tmp_path, K, seed, name):
                           lambda0=p.lambda_, alphapropsd=p.alphapropsd, alphamax=p.alphamax,
                           alphapriora=p.alphapriora, alphapriorb=p.alphapriorb,
                           fpriormean=p.fpriormean, fpriorsd=p.fpriorsd, seed=seed, chain=K)
-    ch = oracle.Chain(m, inp.geno_int8(), inp.nall)
+    ch = oracle.Chain(m, inp.geno16(), inp.nall)
     ch.init()
     total = p.burnin + p.numreps
     for rep in range(total):
@@ -200,3 +200,37 @@ def test_structure_options_match_oracle_chain(tmp_path):
                 na, nb = _numbers(la), _numbers(lb)
                 assert len(na) == len(nb) and np.allclose(na, nb, rtol=0, atol=0.11), (la, lb)
     assert not (tmp_path / "run_3").exists()
+
+
+def test_structure_with_200_alleles_at_a_locus(tmp_path):
+    """A locus with 200 distinct alleles (recoded codes 0..199, beyond the int8
+    range): the command packs the recoded int32 genotypes into bytes
+    (sgpu_pack_byte32) and its result file matches the oracle chain's."""
+    rng = np.random.default_rng(12)
+    N, L = 120, 4
+    lines = []
+    for i in range(N):
+        for c in range(2):
+            al = [1000 + (2 * i + c) % 200,
+                  int(rng.choice([5, 7])) if rng.random() > 0.02 else -9,
+                  int(rng.choice([11, 12, 13])), int(rng.choice([1, 2]))]
+            lines.append(f"ind{i} " + " ".join(str(a) for a in al))
+    (tmp_path / "many.str").write_text("\n".join(lines) + "\n")
+    main = MAIN.replace("NUMINDS    200", f"NUMINDS    {N}").replace(
+        "NUMLOCI    5", f"NUMLOCI    {L}").replace("POPDATA   1", "POPDATA   0").replace(
+        "POPFLAG   1", "POPFLAG   0").replace("MISSING     -999", "MISSING     -9")
+    (tmp_path / "mainparams").write_text(main.format(infile=str(tmp_path / "many.str"),
+                                                     outfile="many"))
+    (tmp_path / "extraparams").write_text(EXTRA.replace("PRINTQHAT 1", "PRINTQHAT 0"))
+    out = _run(tmp_path)
+    assert "Genotypes packed as bytes" in out
+    got = (tmp_path / "many_f").read_text()
+    assert "200 alleles" in got
+    want = _oracle_file(tmp_path, 2, 4242, "oracle")
+    a = got.replace("many", "X").splitlines()
+    b = want.replace(str(tmp_path / "oracle"), "X").splitlines()
+    assert len(a) == len(b)
+    for la, lb in zip(a, b):
+        if la != lb and not la.startswith(("Input File", "DATAFILE")):
+            na, nb = _numbers(la), _numbers(lb)
+            assert len(na) == len(nb) and np.allclose(na, nb, rtol=0, atol=0.11), (la, lb)

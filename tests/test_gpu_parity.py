@@ -322,11 +322,13 @@ def test_tiny_and_odd_shapes_follow_the_oracle(fmt, N, L, K):
 
 
 @pytest.mark.parametrize("N,L,K,A", [(9, 11, 16, 120), (40, 6, 3, 127), (130, 3, 8, 64),
-                                     (9, 11, 20, 127)])
+                                     (9, 11, 20, 127), (12, 9, 4, 200), (7, 5, 20, 255),
+                                     (70, 4, 3, 255)])
 def test_many_alleles_follow_the_oracle(N, L, K, A):
-    """MAXALLELES up to 127 (the int8 input's largest allele index + 1): one locus
-    per LDS chunk in UpdateZ, two in the Metropolis ratio, one or two (locus, pop)
-    groups per UpdateP block."""
+    """MAXALLELES up to 255 (the byte format's largest: alleles 0..254; above 127
+    packed from wider integers by sgpu_pack_byte32): one locus per LDS chunk in
+    UpdateZ (61 KiB at K = 20, A = 255), the Metropolis ratio's staging sized to
+    one locus, one (locus, pop) group per UpdateP block."""
     _follow_oracle("byte", N, L, K, A_byte=A)
 
 

@@ -42,11 +42,11 @@ def _unpack_snp2(words, N, L):
 
 def _unpack_byte(words, N, L):
     w = words.reshape(L, -1)   # [L][Gpad], locus-major
-    out = np.empty((N, 2, L), np.int8)
+    out = np.empty((N, 2, L), np.int16)
     for i in range(N):
         for c in range(2):
             b = (w[:, i >> 1] >> np.uint32(16 * (i & 1) + 8 * c)) & np.uint32(0xff)
-            out[i, c] = np.where(b == 0xff, -9, b).astype(np.int8)
+            out[i, c] = np.where(b == 0xff, -9, b).astype(np.int16)
     return out
 
 
@@ -74,6 +74,28 @@ def test_pack_byte_roundtrip_keeps_copy_order():
     w = words.reshape(77, 64)
     assert np.all(w[:, 11:] == 0xffffffff)            # padded rows are missing
     assert np.all(w[:, 10] >> np.uint32(16) == 0xffff)  # individual 21 of the last row missing
+
+
+def test_pack_byte32_takes_up_to_255_alleles():
+    """sgpu_pack_byte32 packs the reference's int32 Geno with its own missing code:
+    the same words as the int8 packer where both apply, alleles up to 254 beyond."""
+    rng = np.random.default_rng(8)
+    geno, nall = random_geno(rng, 19, 40, 9, miss=0.1)
+    g32 = geno.astype(np.int32)
+    g32[g32 == -9] = -999
+    out = np.empty(sa.packed_words(sa.GENO_BYTE, 19, 40), np.uint32)
+    rc = sa.lib().sgpu_pack_byte32(g32.ctypes.data_as(ctypes.c_void_p), 19, 40, -999,
+                                   out.ctypes.data_as(ctypes.c_void_p), out.size)
+    assert rc == sa.OK and np.array_equal(out, sa.pack(geno, sa.GENO_BYTE)[1])
+    wide, _ = random_geno(rng, 19, 40, 255, miss=0.1)
+    assert wide.dtype == np.int16 and wide.max() > 200
+    fmt, words = sa.pack(wide)
+    assert fmt == sa.GENO_BYTE and np.array_equal(_unpack_byte(words, 19, 40), wide)
+    bad = wide.astype(np.int32)
+    bad[0, 0, 0] = 255                                   # 255 is the missing byte
+    rc = sa.lib().sgpu_pack_byte32(bad.ctypes.data_as(ctypes.c_void_p), 19, 40, -9,
+                                   out.ctypes.data_as(ctypes.c_void_p), out.size)
+    assert rc == sa.ERR_ARG
 
 
 @pytest.mark.parametrize("bad", [(1, 0), (0, -9), (-9, 1), (2, 2)])
