@@ -34,11 +34,14 @@ __device__ __forceinline__ uint32_t lane_select(uint64_t mask, uint32_t v)
     return r;
 }
 
-// v += value (wave-uniform) in the lanes that execute it; asm keeps the
-// compiler from turning the single-lane branch around it into a select.
-__device__ __forceinline__ void collect_add(uint32_t &v, uint32_t value)
+// v_writelane_b32 (the compiler's intrinsic: it moves the lane index into M0
+// itself, as gfx9 encodes the instruction with a scalar data operand)
+extern "C" __device__ int sgpu_writelane(int value, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
+
+// v with lane `lane` replaced by the wave-uniform `value`.
+__device__ __forceinline__ uint32_t write_lane(uint32_t v, uint32_t value, int lane)
 {
-    asm volatile("v_add_u32 %0, %0, %1" : "+v"(v) : "s"(value));
+    return (uint32_t)sgpu_writelane((int)value, lane, (int)v);
 }
 
 // Running product of clamped terms with exponent bookkeeping (CalcLikeInd's
@@ -91,14 +94,30 @@ struct LikeAcc {
 //                        three-input add per individual (a tile holds at most
 //                        65535 copies, so the halves never carry into each other);
 //   per-individual       popcount of the wave ballot of each predicate (SALU),
-//                        added into lane t of a collector register by that lane
-//                        alone, and into LDS once per 16 individuals.
+//                        two individuals' counts packed into one 32-bit value
+//                        (16 bits each: at most 128 copies per wave, 512 per
+//                        tile), written into lane jp of a collector register
+//                        (v_writelane), and added into LDS once per 16 individuals.
 // Counts per population are differences of consecutive "z > k" counts.
 // Q[i,:] (wave uniform) is read through the constant address space, so it is
 // fetched with scalar loads.
 // ---------------------------------------------------------------------------
+// Waves per SIMD asked of the compiler for the K-templated 2-bit UpdateZ where
+// a few registers fewer than its own allocation lift the occupancy a step and
+// the spills stay out of the inner loop; 1 = the compiler's choice.  Measured
+// per K at N=2000, L=100000 (ms per launch, compiler's choice -> this):
+// K=5 0.534 -> 0.525, 7 0.715 -> 0.678, 10 1.027 -> 0.939, 11 1.115 -> 1.084,
+// 14 1.712 -> 1.385, 15 1.910 -> 1.567, 16 2.027 -> 1.907; the same step made
+// K = 4, 6, 8, 9, 12, 13, 17..20 slower (spills inside the loop).
+template <int K>
+constexpr int z_snp2_waves()
+{
+    return K == 5 ? 6 : K == 7 ? 5 : (K == 10 || K == 11) ? 4 : (K >= 14 && K <= 16) ? 3 : 1;
+}
+
 template <int K, bool INIT, bool LIKE, bool ZOUT, bool PF>
-__global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__restrict__ cds,
+__global__ __launch_bounds__(kZLoci) __attribute__((amdgpu_waves_per_eu(z_snp2_waves<K>())))
+void k_update_z_snp2(const ChainDesc *__restrict__ cds,
                                                           const BlockRef *__restrict__ map,
                                                           const uint32_t *__restrict__ repp)
 {
@@ -122,7 +141,9 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     // PFROMPOPFLAGONLY (PF): per word row, bit 2t set when individual 16g+t adds to
     // the allele counts
     const SGPU_G uint32_t *pfmask = g(C.pfmask);
-    extern __shared__ uint32_t s_cnt[];   // [tile individual][K]: valid copies, then z > k
+    // [tile individual pair][K]: valid copies, then z > k; the pair's even individual
+    // in the low 16 bits, the odd one in the high 16 bits
+    extern __shared__ uint32_t s_cnt[];
     const int lane = threadIdx.x & 63;
     const int l = bx * kZLoci + threadIdx.x;   // < Lpad by construction
     const bool lok = l < L;
@@ -131,7 +152,8 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     const int ibase = row0 * 16;
     const int ntile = min((row1 - row0) * 16, N - ibase);
 
-    for (int t = threadIdx.x; t < ntile * K; t += kZLoci) {
+    const int npairs = (ntile + 1) / 2;
+    for (int t = threadIdx.x; t < npairs * K; t += kZLoci) {
         s_cnt[t] = 0;
     }
     __syncthreads();
@@ -169,10 +191,12 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
             const uint32_t vall = ~(lo & hi) & live;
             const uint32_t vmask = vall & pfm;
             Tall += 2u * (uint32_t)__popc(vall);
-            Ttot += 2u * (uint32_t)__popc(vmask);
+            if (PF) {   // without the filter the allele counts take every valid copy
+                Ttot += 2u * (uint32_t)__popc(vmask);
+            }
             A1tot += (uint32_t)__popc(lo & ~hi & vmask) + 2u * (uint32_t)__popc(hi & ~lo & vmask);
         }
-        uint32_t col[K];   // collector: lane t holds the counts of individual 16g+t
+        uint32_t col[K];   // collector: lane jp holds the packed counts of pair jp of the row
 #pragma unroll
         for (int k = 0; k < K; k++) {
             col[k] = 0;
@@ -184,6 +208,11 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
             }
             const u32x4 w = philox4x32_10(rep, (uint32_t)l, (uint32_t)(i0 >> 1),
                                           (INIT ? PH_INITZ : PH_Z) << 24, seed, chain);
+            uint32_t nkp[K];   // the pair's counts, individual i0 low, i0 + 1 high
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                nkp[k] = 0;
+            }
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int i = i0 + h;
@@ -260,29 +289,30 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
                         }
                     }
                 }
-                // park the counts in lane t of the collector: one add per value,
-                // executed by that lane only
-                if (lane == t) {
 #pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        collect_add(col[k], nk[k]);
-                    }
+                for (int k = 0; k < K; k++) {
+                    nkp[k] |= nk[k] << (16 * h);
                 }
                 if (ZOUT && lok) {
                     zout[((size_t)i * 2 + 0) * L + l] = valid ? (int8_t)z0 : (int8_t)-9;
                     zout[((size_t)i * 2 + 1) * L + l] = valid ? (int8_t)z1 : (int8_t)-9;
                 }
             }
+            // park the pair's counts in lane jp of the collector (one VALU per value)
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                col[k] = write_lane(col[k], nkp[k], jp);
+            }
             if (LIKE) {
                 acc.renorm();
             }
         }
-        // add this row's per-individual counts into LDS (lanes 0..15 of each wave)
-        const int t0 = g * 16 - ibase;
-        if (lane < 16 && t0 + lane < ntile) {
+        // add this row's per-pair counts into LDS (lanes 0..7 of each wave)
+        const int p0 = (g * 16 - ibase) / 2;
+        if (lane < 8 && p0 + lane < npairs) {
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                atomicAdd(&s_cnt[(t0 + lane) * K + k], col[k]);
+                atomicAdd(&s_cnt[(p0 + lane) * K + k], col[k]);
             }
         }
         word = next;
@@ -290,7 +320,7 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
 
     if (lok) {
         SGPU_G uint16_t *dst = nApart + ((size_t)by * L + l) * K * A;
-        uint32_t prevT = Ttot, prevA = A1tot;
+        uint32_t prevT = PF ? Ttot : Tall, prevA = A1tot;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const uint32_t tk = (k < K - 1) ? (TA[k] & 0xFFFFu) : 0u;
@@ -307,9 +337,11 @@ __global__ __launch_bounds__(kZLoci) void k_update_z_snp2(const ChainDesc *__res
     // convert "valid, z > 0, ..., z > K-2" into counts per population
     for (int t = threadIdx.x; t < ntile; t += kZLoci) {
         SGPU_G uint16_t *dst = nQpart + ((size_t)bx * N + ibase + t) * K;
-        uint32_t prev = s_cnt[t * K + 0];
+        const uint32_t *src = s_cnt + (t >> 1) * K;
+        const int sh = 16 * (t & 1);
+        uint32_t prev = (src[0] >> sh) & 0xFFFFu;
         for (int k = 0; k < K; k++) {
-            const uint32_t gk = (k < K - 1) ? s_cnt[t * K + 1 + k] : 0u;
+            const uint32_t gk = (k < K - 1) ? (src[1 + k] >> sh) & 0xFFFFu : 0u;
             dst[k] = (uint16_t)(prev - gk);
             prev = gk;
         }
@@ -393,8 +425,15 @@ __device__ __forceinline__ void byte_copy(int x, bool valid, bool count, uint32_
     zout = z;
 }
 
+// Waves per SIMD asked for the byte-format UpdateZ (as z_snp2_waves), measured
+// on microsatellites (N=2000, L=20000, 2-16 alleles; ms per launch): K=12 0.350
+// -> 0.341, K=17 0.508 -> 0.452; the same step at K=5 and 8 was slower or even.
+template <int K>
+constexpr int z_byte_waves() { return K == 12 ? 4 : K == 17 ? 3 : 1; }
+
 template <int K, bool INIT, bool LIKE, bool ZOUT, bool PF>
-__global__ __launch_bounds__(kByteRows) void k_update_z_byte(const ChainDesc *__restrict__ cds,
+__global__ __launch_bounds__(kByteRows) __attribute__((amdgpu_waves_per_eu(z_byte_waves<K>())))
+void k_update_z_byte(const ChainDesc *__restrict__ cds,
                                                              const BlockRef *__restrict__ map,
                                                              const uint32_t *__restrict__ repp)
 {

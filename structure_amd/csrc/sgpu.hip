@@ -850,7 +850,7 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         }
         d.n_it = (int)cdiv(d.G, rows);
         d.rows_per_tile = (int)cdiv(d.G, d.n_it);   // even tiles
-        h.z_lds = (size_t)rows * 16 * K * sizeof(uint32_t);
+        h.z_lds = (size_t)rows * 8 * K * sizeof(uint32_t);   // counts of individual pairs
     } else {
         // one individual pair per lane, byte_rows (64, 128 or 256) pairs per block,
         // locus tiles from batch_tiles; a chunk of loci's K x A frequencies (FP64)
