@@ -45,6 +45,7 @@ def sweep_bytes(N, L, K, A):
 
 
 CPU_SHARE = 16   # host cores one GPU's runs may use on the benchmark host
+MIN_WARM = 30    # warm-up sweeps of the single-chain lines, at least
 
 
 def cpu_info():
@@ -153,7 +154,7 @@ def batched_ksweep(d, N, L, seed, sweeps, warmup, kmax=10, reps=10):
     Returns the line and, per K, the final (Q, P, lnP) of replicate 0 for the
     error check against the oracle."""
     import structure_amd as sa
-    warmup = max(warmup, 2)
+    warmup = max(warmup, 10)
     models = [sa.make_model(N, L, K, 2, freqscorr=1, inferalpha=1, inferlambda=1, metrofreq=10,
                             burnin=0, numreps=sweeps, seed=seed + r, chain=K)
               for K in range(1, kmax + 1) for r in range(reps)]
@@ -548,8 +549,10 @@ def main():
     print(f"bench: synthetic data N={N} L={L} in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
           flush=True)
     # every sweep collects (BURNIN=0): the warm-up, at least a Metropolis sweep (rep 0)
-    # and a Gibbs one (rep 1), captures both graphs that the timed sweeps replay
-    warm = max(args.warmup, 2)
+    # and a Gibbs one (rep 1), captures both graphs that the timed sweeps replay.  It
+    # runs at least MIN_WARM sweeps: right after start-up the first 20 sweeps run 6%
+    # slower than the next (clocks and caches warming), so fewer would time the ramp.
+    warm = max(args.warmup, MIN_WARM)
     model = sa.make_model(N, L, K, 2, freqscorr=1, inferalpha=1, inferlambda=1, metrofreq=10,
                           burnin=0, numreps=args.steps, seed=args.seed, chain=0)
     tuning = None
