@@ -614,6 +614,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": warm,
+        "warmup_requested": args.warmup,
         "ms_per_step": round(1e3 * dt / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
