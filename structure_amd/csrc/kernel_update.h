@@ -17,6 +17,9 @@ namespace sgpu {
 
 constexpr int kNT = 256;   // threads per block for the update kernels
 
+__device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint32_t rep,
+                                   int nparts);
+
 // The chain and block coordinates of this block (uniform: scalar loads).
 #define SGPU_BLOCK_PROLOGUE()                          \
     const BlockRef br_ = map[blockIdx.x];              \
@@ -85,11 +88,17 @@ __device__ __forceinline__ void wave_col_sums(const SGPU_G double *x, int n, int
 // individual-tile partials, coalesced), one gamma draw per thread, then the
 // group's sum in allele order from LDS and LogRDirichletDisc's rule
 // (ran.c:1247-1272) including the sum < UNDERFLO uniform fallback.
-// Fused: PSum += P (output.c:673-677) when collecting.
+// Fused: PSum += P (output.c:673-677) when collecting; with `lam`, the block's
+// partial log sums of the lambda update (UpdateLambda.c:8-102) over loci with
+// more than one allele, one row of lam_part per block: sum of log eps (F
+// model: the epsilon this pass reads is the one lambda sees, as nothing writes
+// it in between), of log P of the new frequencies, or per pop of log P
+// (pop-specific lambda).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ cds,
                                                    const BlockRef *__restrict__ map,
-                                                   const uint32_t *__restrict__ repp, int collect)
+                                                   const uint32_t *__restrict__ repp, int collect,
+                                                   int lam)
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
@@ -122,7 +131,7 @@ __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ 
     const int na = live ? nall[l] : 0;
     const bool act = aa < na;
     const size_t pos = ((size_t)l * m.K + k) * A + aa;
-    double x = 0.0;
+    double x = 0.0, ev = 1.0;
     if (act) {
         // the partials are independent loads: eight in flight, exact integer sums
         const size_t stride = (size_t)m.L * m.K * A;
@@ -149,25 +158,45 @@ __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ 
         }
         const int n = (n0 + n1) + (n2 + n3);
         const double Fk = F[k];
-        const double par = m.freqscorr ? eps[(size_t)l * A + aa] * (1.0 - Fk) / Fk + (double)n
-                                       : lambda[k] + (double)n;
+        ev = m.freqscorr ? eps[(size_t)l * A + aa] : 1.0;
+        const double par = m.freqscorr ? ev * (1.0 - Fk) / Fk + (double)n : lambda[k] + (double)n;
         x = gamma_draw(par, m.seed, m.chain, rep, (uint32_t)l, ((uint32_t)k << 16) | (uint32_t)aa,
                        PH_P);
     }
     s_x[threadIdx.x] = x;
     __syncthreads();
-    if (!act) {
-        return;
+    double p = 1.0;
+    if (act) {
+        double sum = 0.0;
+        const int g0 = (int)threadIdx.x - aa;
+        for (int j = 0; j < na; j++) {   // allele order, as the host loop adds them
+            sum += s_x[g0 + j];
+        }
+        p = sum < kUnderflo ? 1.0 / (double)na : x / sum;
+        P[pos] = p;
+        if (collect) {
+            PSum[pos] += p;
+        }
     }
-    double sum = 0.0;
-    const int g0 = (int)threadIdx.x - aa;
-    for (int j = 0; j < na; j++) {   // allele order, as the host loop adds them
-        sum += s_x[g0 + j];
-    }
-    const double p = sum < kUnderflo ? 1.0 / (double)na : x / sum;
-    P[pos] = p;
-    if (collect) {
-        PSum[pos] += p;
+    if (lam && m.inferlambda) {   // block-uniform
+        __shared__ double s_red[kNT / 64];
+        SGPU_G double *part = g(C.lam_part);
+        const bool use = act && na > 1;
+        if (m.popspecificlambda) {
+            const double v = use ? log_d(p) : 0.0;
+            for (int s = 0; s < m.K; s++) {
+                const double t = block_sum<kNT>(k == s ? v : 0.0, s_red);
+                if (threadIdx.x == 0) {
+                    part[(size_t)bx * m.K + s] = t;
+                }
+            }
+        } else {
+            const double v = !use ? 0.0 : (m.freqscorr ? (k == 0 ? log_d(ev) : 0.0) : log_d(p));
+            const double t = block_sum<kNT>(v, s_red);
+            if (threadIdx.x == 0) {
+                part[bx] = t;
+            }
+        }
     }
 }
 
@@ -250,11 +279,9 @@ constexpr int kRowSplit = 16;      // second-level partials of the ancestry coun
 // UpdateZ ([n_lt][N*K]) are folded into kRowSplit partials; chain grid (flat
 // chunks, kRowSplit), each thread sums every kRowSplit-th row of one flat
 // index with four independent accumulators.
-__global__ __launch_bounds__(kNT) void k_fold_rows(const ChainDesc *__restrict__ cds,
-                                                   const BlockRef *__restrict__ map)
+__device__ __forceinline__ void fold_rows_block(const ChainDesc &C, const ModelArgs &m, int bx,
+                                                int by)
 {
-    SGPU_BLOCK_PROLOGUE();
-    const int by = (int)br_.by;
     const SGPU_G uint16_t *part = g(C.nQpart);
     SGPU_G int32_t *out = g(C.nQfold);
     const int nrows = C.n_lt;
@@ -277,6 +304,13 @@ __global__ __launch_bounds__(kNT) void k_fold_rows(const ChainDesc *__restrict__
     out[(size_t)by * n + f] = (a0 + a1) + (a2 + a3);
 }
 
+__global__ __launch_bounds__(kNT) void k_fold_rows(const ChainDesc *__restrict__ cds,
+                                                   const BlockRef *__restrict__ map)
+{
+    SGPU_BLOCK_PROLOGUE();
+    fold_rows_block(C, m, bx, (int)br_.by);
+}
+
 // Gibbs update of Q (UpdateQAdmixture, UpdateQ.c:387-424; RDirichlet,
 // ran.c:348-359): one thread per (individual, pop), a block holding whole
 // individuals (K * (256 / K) threads).  Each thread sums its folded ancestry
@@ -293,6 +327,10 @@ __global__ __launch_bounds__(kNT) void k_update_q_gibbs(const ChainDesc *__restr
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    if (br_.role == kRoleLambda) {   // the chain's lambda decision, from UpdateP's partials
+        lambda_final_block(C, m, rep, C.p_blocks);
+        return;
+    }
     const SGPU_G double *alpha = g(C.alpha);
     SGPU_G double *Q = g(C.Q);
     SGPU_G double *QSum = g(C.QSum);
@@ -425,6 +463,10 @@ __global__ __launch_bounds__(kNT) void k_metro_propose(const ChainDesc *__restri
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    if (br_.role == kRoleLambda) {   // the chain's lambda decision, from UpdateP's partials
+        lambda_final_block(C, m, rep, C.p_blocks);
+        return;
+    }
     __shared__ double s_x[kNT];
     const int K = m.K;
     const int ipb = kNT / K;   // one thread per (individual, pop), whole individuals per block
@@ -796,16 +838,15 @@ __global__ __launch_bounds__(kNT) void k_lambda_partial(const ChainDesc *__restr
     }
 }
 
-__global__ __launch_bounds__(kNT) void k_lambda_finalize(const ChainDesc *__restrict__ cds,
-                                                          const BlockRef *__restrict__ map,
-                                                          const uint32_t *__restrict__ repp)
+// The lambda decision of one chain by a whole block of kNT threads: sums the
+// nparts rows of partials, then thread 0 runs the proposal(s).
+__device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint32_t rep,
+                                   int nparts)
 {
-    SGPU_BLOCK_PROLOGUE();
-    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const SGPU_G double *part = g(C.lam_part);
     const SGPU_G int32_t *ahist = g(C.ahist);
     SGPU_G double *lambda = g(C.lambda);
-    const int nparts = C.l_blocks, nhist = C.nhist;
+    const int nhist = C.nhist;
     __shared__ double s_red[kNT / 64];
     __shared__ double s_stat[kMaxK];
     const int nsums = m.popspecificlambda ? m.K : 1;
@@ -854,6 +895,14 @@ __global__ __launch_bounds__(kNT) void k_lambda_finalize(const ChainDesc *__rest
     }
 }
 
+__global__ __launch_bounds__(kNT) void k_lambda_finalize(const ChainDesc *__restrict__ cds,
+                                                          const BlockRef *__restrict__ map,
+                                                          const uint32_t *__restrict__ repp)
+{
+    SGPU_BLOCK_PROLOGUE();
+    lambda_final_block(C, m, *repp, C.l_blocks);
+}
+
 // Fst proposal of group pop1 (UpdateFst.c:160-166): F + FPRIORSD * N(0,1).
 __device__ __forceinline__ double fst_proposal(const ModelArgs &m, double oldf, int pop1,
                                                uint32_t rep, uint32_t *uw)
@@ -865,18 +914,147 @@ __device__ __forceinline__ double fst_proposal(const ModelArgs &m, double oldf, 
     return oldf + m.fpriorsd * box_muller(w.x, w.y);
 }
 
+// Sum of x[b * stride + off] over b = first, first + step, ... < n by one lane,
+// in eight interleaved accumulators (eight loads in flight) added in a fixed tree.
+__device__ __forceinline__ double lane_sum8(const SGPU_G double *x, int n, int stride, int off,
+                                            int first, int step)
+{
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int b = first;
+    for (; b + 7 * step < n; b += 8 * step) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            a[u] += x[(size_t)(b + u * step) * stride + off];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        if (b + u * step < n) {
+            a[u] += x[(size_t)(b + u * step) * stride + off];
+        }
+    }
+    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
+// Column sums of an [n][ncol] row-major table (ncol <= kMaxK + 1) by a block of
+// NT threads.  With ncol <= NT / 64 each column gets NT / 64 / ncol waves, each
+// wave a strided share of the rows reduced by its shuffle tree, and a column's
+// wave sums are added in wave order; otherwise wave w takes columns w, w + NT / 64,
+// ... whole.  The order is fixed, so the sums do not depend on scheduling.
+// out[c] (LDS) is valid in every thread on return.
+template <int NT>
+__device__ void block_col_sums(const SGPU_G double *x, int n, int ncol, double *s_part /* NT/64 */,
+                               double *out /* kMaxK + 1 */)
+{
+    constexpr int NW = NT / 64;
+    const int lane = (int)(threadIdx.x & 63), wid = (int)(threadIdx.x >> 6);
+    if (ncol <= NW) {
+        const int wpc = NW / ncol;   // waves per column
+        const int c = wid / wpc, share = wid - c * wpc;
+        double v = 0.0;
+        if (c < ncol) {
+            v = wave_sum(lane_sum8(x, n, ncol, c, share * 64 + lane, wpc * 64));
+        }
+        if (lane == 0) {
+            s_part[wid] = v;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < ncol) {
+            double t = 0.0;
+            for (int s = 0; s < wpc; s++) {
+                t += s_part[threadIdx.x * wpc + s];
+            }
+            out[threadIdx.x] = t;
+        }
+    } else {
+        for (int c = wid; c < ncol; c += NW) {
+            const double v = wave_sum(lane_sum8(x, n, ncol, c, lane, 64));
+            if (lane == 0) {
+                out[c] = v;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// lnP(D) of the last UpdateZ as the sum of its tile partials: share s of ns
+// contiguous shares, by one wave (valid in lane 0).
+__device__ __forceinline__ double like_share(const ChainDesc &C, int s, int ns)
+{
+    const int n = C.n_it * C.n_lt;
+    const int b0 = (int)((long long)n * s / ns), b1 = (int)((long long)n * (s + 1) / ns);
+    return wave_sum(lane_sum8(g(C.likepart) + b0, b1 - b0, 1, 0, (int)(threadIdx.x & 63), 64));
+}
+
+// Flags of the epsilon launch.
+constexpr int kEpsCollect = 1, kEpsFst = 2, kEpsAlpha = 4, kEpsLike = 8;
+
+// The kRoleScalars block of a chain in the epsilon launch (kNT threads): wave 0
+// runs the alpha update (it reads only Q, whose update closed before UpdateZ,
+// and alpha), waves 1-3 sum thirds of the lnP(D) tile partials; then wave 0
+// adds lambda and alpha (both final for the sweep: the lambda decision ran in
+// the Q-update launch) and the lnP accumulators into the posterior sums
+// (output.c:50-101, UpdateSums :654-707).  F and its sum follow in the tail.
+__device__ void scalars_block(const ChainDesc &C, const ModelArgs &m, uint32_t rep, int flags)
+{
+    __shared__ double s_like[3];
+    const int wid = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    if (wid == 0 && (flags & kEpsAlpha) && m.inferalpha) {
+        alpha_wave(C, m, rep);
+    }
+    if (wid >= 1 && (flags & kEpsLike)) {
+        const double v = like_share(C, wid - 1, 3);
+        if (lane == 0) {
+            s_like[wid - 1] = v;
+        }
+    }
+    __syncthreads();
+    if (!(flags & kEpsCollect) || wid != 0) {
+        return;
+    }
+    if (lane < m.K) {
+        g(C.lambdaSum)[lane] += g(C.lambda)[lane];
+        g(C.alphaSum)[lane] += g(C.alpha)[lane];
+    }
+    if ((flags & kEpsLike) && lane == 0) {
+        const double like = (s_like[0] + s_like[1]) + s_like[2];
+        SGPU_G double *like3 = g(C.like3);
+        like3[0] = like;
+        like3[1] += like;
+        like3[2] += like * like;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Epsilon (NonIndependenceUpdateEpsilon, UpdateEpsilon.c:71-145).  One thread
 // per (locus, pop) term; a block holds whole loci (K * (256 / K) threads) so
 // the K terms of a locus meet in LDS and are summed there in the reference's
-// order.  Fused: SumEpsilon += eps (output.c:684-687) when collecting.
+// order.  Fused: SumEpsilon += eps (output.c:684-687) with kEpsCollect, and
+// with kEpsFst the allele terms of the Fst log-ratio over the updated eps
+// (FlikeFreqs(new) - FlikeFreqs(old), UpdateFst.c:22-51): thread (l, k) adds
+// pop k's terms at locus l to its proposal group (group k, or group 0 with
+// ONEFST), taking the old-F lgamma values and the log P of the two alleles
+// the epsilon move touched from the move itself (the same arguments, hence the
+// same values), and the block writes one partial per group.  The proposals are
+// drawn here as in the Fst decision (same stream, same values).
+// Blocks of other roles ride in the same launch: kRoleFold blocks fold this
+// sweep's ancestry-count partials for the next sweep's Gibbs update, and one
+// kRoleScalars block per chain runs alpha, the lnP(D) sum and the scalar sums.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict__ cds,
                                                      const BlockRef *__restrict__ map,
-                                                     const uint32_t *__restrict__ repp, int collect)
+                                                     const uint32_t *__restrict__ repp, int flags)
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    if (br_.role == kRoleFold) {
+        fold_rows_block(C, m, bx, (int)br_.by);
+        return;
+    }
+    if (br_.role == kRoleScalars) {
+        scalars_block(C, m, rep, flags);
+        return;
+    }
     const SGPU_G int32_t *nall = g(C.nall);
     const SGPU_G double *P = g(C.P);
     const SGPU_G double *F = g(C.F);
@@ -884,17 +1062,24 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
     SGPU_G double *eps = g(C.eps);
     SGPU_G double *epsSum = g(C.epsSum);
     __shared__ double s_term[6][kNT];   // the six terms of each (locus, pop), summed in order below
+    __shared__ double s_f[kNT];         // Fst terms of each (locus, pop)
+    __shared__ double s_newf[kMaxK];    // this sweep's Fst proposals
+    __shared__ int s_acc[kNT];          // per locus of the block: the move was accepted
     const int K = m.K;
     const int lpb = kNT / K;                       // loci per block
     const int j = threadIdx.x / K, k = threadIdx.x % K;
     const int l = bx * lpb + j;
     const bool live = j < lpb && l < m.L;
     const int na = live ? nall[l] : 0;
+    const bool fst = (flags & kEpsFst) != 0;   // block-uniform
     SGPU_G double *e = eps + (size_t)(live ? l : 0) * m.A;
     int a1 = 0, a2 = 0;
     double d = 0.0, e1 = 0.0, e2 = 0.0;
     bool inrange = false;
     u32x4 w{0u, 0u, 0u, 0u};
+    if (fst && (int)threadIdx.x < (m.onefst ? 1 : K)) {
+        s_newf[threadIdx.x] = fst_proposal(m, F[threadIdx.x], threadIdx.x, rep, nullptr);
+    }
     if (na > 1) {
         w = philox4x32_10(rep, (uint32_t)l, 0u, PH_EPS << 24, m.seed, m.chain);
         a1 = (int)(uz(w.x) * (double)na);
@@ -907,21 +1092,25 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
         e2 = e[a2];
         inrange = (e1 + d < 1.0) && (e2 - d > 0.0);
     }
-    if (bx == 0 && (int)threadIdx.x < (m.onefst ? 1 : m.K)) {
-        // this sweep's Fst proposals (F does not change before the Fst update)
-        g(C.fprop)[threadIdx.x] = fst_proposal(m, F[threadIdx.x], threadIdx.x, rep, nullptr);
-    }
+    const double frac = (1.0 - F[k]) / F[k];
+    double lg0 = 0.0, lg1 = 0.0, lg2 = 0.0, lg3 = 0.0, lp1 = 0.0, lp2 = 0.0;
     if (inrange) {
-        const double frac = (1.0 - F[k]) / F[k];
-        s_term[0][threadIdx.x] = lgamma_lanczos(frac * e1);
-        s_term[1][threadIdx.x] = lgamma_lanczos(frac * e2);
-        s_term[2][threadIdx.x] = lgamma_lanczos(frac * (e1 + d));
-        s_term[3][threadIdx.x] = lgamma_lanczos(frac * (e2 - d));
-        s_term[4][threadIdx.x] = frac * d * log_d(P[((size_t)l * K + k) * m.A + a1]);
-        s_term[5][threadIdx.x] = frac * d * log_d(P[((size_t)l * K + k) * m.A + a2]);
+        lg0 = lgamma_lanczos(frac * e1);
+        lg1 = lgamma_lanczos(frac * e2);
+        lg2 = lgamma_lanczos(frac * (e1 + d));
+        lg3 = lgamma_lanczos(frac * (e2 - d));
+        lp1 = log_d(P[((size_t)l * K + k) * m.A + a1]);
+        lp2 = log_d(P[((size_t)l * K + k) * m.A + a2]);
+        s_term[0][threadIdx.x] = lg0;
+        s_term[1][threadIdx.x] = lg1;
+        s_term[2][threadIdx.x] = lg2;
+        s_term[3][threadIdx.x] = lg3;
+        s_term[4][threadIdx.x] = frac * d * lp1;
+        s_term[5][threadIdx.x] = frac * d * lp2;
     }
     __syncthreads();
     if (live && k == 0) {
+        int acc = 0;
         if (inrange) {
             double sum = 0.0;
             for (int kk = 0; kk < K; kk++) {   // the reference's accumulation order
@@ -942,29 +1131,83 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
             if (uo(w.w) < exp(sum)) {
                 e[a1] = e1 + d;
                 e[a2] = e2 - d;
+                acc = 1;
             }
         }
-        if (collect) {
+        s_acc[j] = acc;
+        if (flags & kEpsCollect) {
             for (int aa = 0; aa < na; aa++) {
                 epsSum[(size_t)l * m.A + aa] += e[aa];
             }
         }
     }
+    if (!fst) {
+        return;
+    }
+    __syncthreads();
+    double v = 0.0;
+    const int grp = m.onefst ? 0 : k;
+    const double newf = s_newf[grp];
+    if (live && na > 0 && newf > 0.0 && newf < 1.0) {
+        const double oldf = F[grp];
+        const double nfr = (1.0 - newf) / newf;
+        const double ofr = (1.0 - oldf) / oldf;
+        const bool acc = inrange && s_acc[j] != 0;
+        const bool reuse = inrange && ofr == frac;   // the move's lgamma arguments are ours
+        const SGPU_G double *pl = P + ((size_t)l * K + k) * m.A;
+        for (int aa = 0; aa < na; aa++) {
+            double ep, lp, lo;
+            if (reuse && aa == a1) {
+                ep = acc ? e1 + d : e1;
+                lp = lp1;
+                lo = acc ? lg2 : lg0;
+            } else if (reuse && aa == a2) {
+                ep = acc ? e2 - d : e2;
+                lp = lp2;
+                lo = acc ? lg3 : lg1;
+            } else {
+                ep = (acc && aa == a1) ? e1 + d : ((acc && aa == a2) ? e2 - d : e[aa]);
+                lp = log_d(pl[aa]);
+                lo = lgamma_lanczos(ofr * ep);
+            }
+            v += (nfr * ep * lp - lgamma_lanczos(nfr * ep)) - (ofr * ep * lp - lo);
+        }
+    }
+    // per group, a pairwise tree over the block's loci
+    s_f[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = 1; st < lpb; st *= 2) {
+        if (j < lpb && j % (2 * st) == 0 && j + st < lpb) {
+            s_f[threadIdx.x] += s_f[threadIdx.x + st * K];
+        }
+        __syncthreads();
+    }
+    SGPU_G double *part = g(C.fst_part);
+    if (m.onefst) {
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int kk = 0; kk < K; kk++) {
+                t += s_f[kk];
+            }
+            part[bx] = t;
+        }
+    } else if ((int)threadIdx.x < K) {
+        part[(size_t)bx * K + threadIdx.x] = s_f[threadIdx.x];
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Fst (UpdateFst.c:142-204).  Pass 1 (grid: locus blocks x proposal groups)
-// takes its group's proposal (drawn by the epsilon pass of the sweep, or once
-// per block when called alone) and sums, per locus, the allele terms of
-// FlikeFreqs(new) - FlikeFreqs(old) (UpdateFst.c:22-51); pass 2 (one wave per
-// chain) adds the prior ratio (FPriorDiff, :10-18) and the per-locus
-// lgamma(frac) terms and decides each group in its own lane.  ONEFST: one
-// proposal, likelihood summed over all pops.
+// Fst (UpdateFst.c:142-204).  The per-locus allele terms of
+// FlikeFreqs(new) - FlikeFreqs(old) (UpdateFst.c:22-51) come in block partials,
+// per proposal group: from the epsilon pass of the sweep (above), or, called
+// alone, from k_fst_partial (grid: locus blocks x proposal groups, proposal
+// drawn once per block).  The decision (one lane per group) adds the prior
+// ratio (FPriorDiff, :10-18) and the per-locus lgamma(frac) terms.  ONEFST:
+// one proposal, likelihood summed over all pops.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void k_fst_partial(const ChainDesc *__restrict__ cds,
                                                       const BlockRef *__restrict__ map,
-                                                      const uint32_t *__restrict__ repp,
-                                                      int use_fprop)
+                                                      const uint32_t *__restrict__ repp)
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
@@ -977,19 +1220,12 @@ __global__ __launch_bounds__(kNT) void k_fst_partial(const ChainDesc *__restrict
     __shared__ double s_f[2];
     const int numpops1 = m.onefst ? 1 : m.K;
     const int p1 = (int)br_.by;   // the proposal group of this block
-    double oldf, newf;
-    if (use_fprop) {              // drawn by the epsilon pass of this sweep
-        oldf = F[p1];
-        newf = g(C.fprop)[p1];
-    } else {                      // drawn here, once per block
-        if (threadIdx.x == 0) {
-            s_f[0] = F[p1];
-            s_f[1] = fst_proposal(m, s_f[0], p1, rep, nullptr);
-        }
-        __syncthreads();
-        oldf = s_f[0];
-        newf = s_f[1];
+    if (threadIdx.x == 0) {
+        s_f[0] = F[p1];
+        s_f[1] = fst_proposal(m, s_f[0], p1, rep, nullptr);
     }
+    __syncthreads();
+    const double oldf = s_f[0], newf = s_f[1];
     const int l = bx * kNT + threadIdx.x;
     const int na = (l < m.L) ? nall[l] : 0;
     double v = 0.0;
@@ -1020,43 +1256,54 @@ __global__ __launch_bounds__(kNT) void k_fst_partial(const ChainDesc *__restrict
     }
 }
 
-__device__ void fst_final_wave(const ChainDesc &C, const ModelArgs &m, uint32_t rep)
+// The decision of proposal group p1 given its summed partials (lane p1 of a
+// wave); with collect, FstSum += F for the group's pops (UpdateSums,
+// output.c:654-707), from the values this lane itself settles.
+__device__ __forceinline__ void fst_decide(const ChainDesc &C, const ModelArgs &m, uint32_t rep,
+                                           int p1, double stat, bool collect)
 {
-    const SGPU_G double *part = g(C.fst_part);
     SGPU_G double *F = g(C.F);
-    const int nparts = C.l_blocks, nloci_nonzero = C.nloci_nonzero;
-    const int numpops1 = m.onefst ? 1 : m.K;
-    // the proposal groups are independent (each reads and writes its own F), so
-    // lane p1 decides group p1 with the group's sum of block partials
-    const int p1 = (int)(threadIdx.x & 63);
-    double sums[kMaxK + 1];
-    wave_col_sums(part, nparts, numpops1, sums);
-    double stat = 0.0;
-#pragma unroll
-    for (int q = 0; q < kMaxK; q++) {
-        stat = p1 == q ? sums[q] : stat;
-    }
-    if (p1 >= numpops1) {
-        return;
-    }
     const double mu = m.fpriormean, sd = m.fpriorsd;
     uint32_t uw;
     const double oldf = F[p1];
     const double newf = fst_proposal(m, oldf, p1, rep, &uw);
+    bool acc = false;
     if (newf > 0.0 && newf < 1.0) {
         double lpd = ((mu * mu / (sd * sd) - 1) * log_d(newf / oldf) + (oldf - newf) * mu / (sd * sd));
         const double nfr = (1.0 - newf) / newf;
         const double ofr = (1.0 - oldf) / oldf;
         const int ngroup = m.onefst ? m.K : 1;
-        lpd += (double)ngroup * (double)nloci_nonzero * (lgamma_lanczos(nfr) - lgamma_lanczos(ofr));
+        lpd += (double)ngroup * (double)C.nloci_nonzero * (lgamma_lanczos(nfr) - lgamma_lanczos(ofr));
         lpd += stat;
-        const bool acc = (lpd >= 0.0) || (uo(uw) < exp(lpd));
+        acc = (lpd >= 0.0) || (uo(uw) < exp(lpd));
+    }
+    const int p2end = m.onefst ? m.K : p1 + 1;
+    for (int p2 = p1; p2 < p2end; p2++) {
+        const double f = acc ? newf : F[p2];
         if (acc) {
-            const int p2end = m.onefst ? m.K : p1 + 1;
-            for (int p2 = p1; p2 < p2end; p2++) {
-                F[p2] = newf;
-            }
+            F[p2] = newf;
         }
+        if (collect) {
+            g(C.FstSum)[p2] += f;
+        }
+    }
+}
+
+__device__ void fst_final_wave(const ChainDesc &C, const ModelArgs &m, uint32_t rep, int nparts)
+{
+    const int numpops1 = m.onefst ? 1 : m.K;
+    // the proposal groups are independent (each reads and writes its own F), so
+    // lane p1 decides group p1 with the group's sum of block partials
+    const int p1 = (int)(threadIdx.x & 63);
+    double sums[kMaxK + 1];
+    wave_col_sums(g(C.fst_part), nparts, numpops1, sums);
+    double stat = 0.0;
+#pragma unroll
+    for (int q = 0; q < kMaxK; q++) {
+        stat = p1 == q ? sums[q] : stat;
+    }
+    if (p1 < numpops1) {
+        fst_decide(C, m, rep, p1, stat, false);
     }
 }
 
@@ -1066,13 +1313,13 @@ __global__ __launch_bounds__(kNT) void k_fst_finalize(const ChainDesc *__restric
 {
     SGPU_BLOCK_PROLOGUE();
     if (threadIdx.x < 64) {
-        fst_final_wave(C, m, *repp);
+        fst_final_wave(C, m, *repp, C.l_blocks);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Data-collection tail (output.c:50-101, UpdateSums :654-707): scalar sums
-// and lnP(D) from the UpdateZ partials (one block per chain, fixed order).
+// Data-collection pass called alone (output.c:50-101, UpdateSums :654-707):
+// scalar sums and lnP(D) from the UpdateZ partials (one block per chain).
 // ---------------------------------------------------------------------------
 // lnP(D) of the last UpdateZ is the sum of its tile partials, split in two
 // halves summed by two waves (lane 0 of each), the halves then added in order.
@@ -1123,38 +1370,30 @@ __global__ __launch_bounds__(128) void k_collect(const ChainDesc *__restrict__ c
     }
 }
 
-// The single-block passes that close a sweep, one launch (one block per chain,
-// one wave each): alpha (it reads only Q and alpha, so it may follow Z, lambda,
-// epsilon and Fst), the Fst decision, and the data collection.  With kTailAdvance the block that
+// The pass that closes a sweep, one block of kTailNT threads per chain with
+// the F model: the Fst decision from the epsilon pass's partials (all 16 waves
+// sum them) and, when collecting, FstSum.  With kTailAdvance the block that
 // finishes last (counted in *done, which it resets) advances the device sweep
-// index: every block has used the index by then, so a graph replay may end here.
-constexpr int kTailAlpha = 1, kTailFst = 2, kTailCollect = 4, kTailLike = 8, kTailAdvance = 16;
+// index: every block of the sweep has used the index by then, so a graph
+// replay may end here.
+constexpr int kTailNT = 1024;
+constexpr int kTailFst = 1, kTailCollect = 2, kTailAdvance = 4;
 
-__global__ __launch_bounds__(kNT) void k_sweep_tail(const ChainDesc *__restrict__ cds,
-                                                     const BlockRef *__restrict__ map,
-                                                     uint32_t *__restrict__ repd,
-                                                     uint32_t *__restrict__ done, int flags)
+__global__ __launch_bounds__(kTailNT) void k_sweep_tail(const ChainDesc *__restrict__ cds,
+                                                         const BlockRef *__restrict__ map,
+                                                         uint32_t *__restrict__ repd,
+                                                         uint32_t *__restrict__ done, int flags)
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repd;
-    const int wid = (int)(threadIdx.x >> 6);
-    __shared__ double s_half[2];
-    // independent waves: alpha, the Fst decision, and the two halves of the lnP sum
-    if (wid == 0 && (flags & kTailAlpha) && m.inferalpha) {
-        alpha_wave(C, m, rep);
-    }
-    if (wid == 1 && (flags & kTailFst) && m.freqscorr) {
-        fst_final_wave(C, m, rep);
-    }
-    if (wid >= 2 && (flags & kTailLike)) {
-        const double v = like_half(C, wid - 2);
-        if ((threadIdx.x & 63) == 0) {
-            s_half[wid - 2] = v;
+    __shared__ double s_part[kTailNT / 64];
+    __shared__ double s_stat[kMaxK + 1];
+    if ((flags & kTailFst) && m.freqscorr) {
+        const int numpops1 = m.onefst ? 1 : m.K;
+        block_col_sums<kTailNT>(g(C.fst_part), C.e_blocks, numpops1, s_part, s_stat);
+        if ((int)threadIdx.x < numpops1) {
+            fst_decide(C, m, rep, (int)threadIdx.x, s_stat[threadIdx.x], (flags & kTailCollect) != 0);
         }
-    }
-    __syncthreads();   // alpha, F and the lnP sum are read by wave 0 below
-    if ((flags & kTailCollect) && threadIdx.x < 64) {
-        collect_scalars(C, m, (flags & kTailLike) != 0, s_half[0] + s_half[1]);
     }
     if (flags & kTailAdvance) {
         __syncthreads();

@@ -127,6 +127,13 @@ struct sgpu_ctx {
     std::vector<KGroup> z_groups, metro_groups;
     Plan p_update_p, p_fold, p_gibbs, p_logq, p_accept, p_alpha, p_lam_part,
         p_lam_fin, p_eps, p_fst_part, p_fst_fin, p_collect, p_init_eps;
+    // the fused launches of a sweep: the Q-update launch with each chain's lambda
+    // decision block; the epsilon launch with the scalar blocks, epsilon + Fst
+    // blocks and count-fold blocks; the Fst decision (F-model chains)
+    Plan p_gibbs_sw, p_postz, p_tail;
+    // nQfold holds the fold of the current nQpart (false after UpdateZ or a host
+    // write of the ancestry counts, until a fold runs)
+    bool fold_fresh = false;
     std::vector<Plan> p_logq_chain;   // per chain, after a host write of Q
     // device sweep index read by every kernel
     uint32_t *repd = nullptr;
@@ -209,12 +216,13 @@ constexpr int kXcds = 8;
 // list order, so blocks near each other in the list share its L2.
 enum PlanOrder { kXFast = 0, kXcdRuns = 2 };
 
-// Block map over the chains selected by sel; dims(chain) gives the chain's
-// own grid (nx, ny).  Blocks are listed chain by chain, in the given order.
+// Appends the blocks of the chains selected by sel to v, chain by chain: dims(chain)
+// gives the chain's own grid (nx, ny), listed row by row (x fastest), each block
+// with the given role.
 template <class Sel, class Dims>
-int build_plan(sgpu_ctx *c, Plan &p, Sel sel, Dims dims, int order = kXFast)
+void add_blocks(const sgpu_ctx *c, std::vector<BlockRef> &v, Sel sel, Dims dims,
+                uint32_t role = kRoleMain)
 {
-    std::vector<BlockRef> v;
     for (size_t ci = 0; ci < c->chains.size(); ci++) {
         if (!sel(c->chains[ci])) {
             continue;
@@ -222,10 +230,26 @@ int build_plan(sgpu_ctx *c, Plan &p, Sel sel, Dims dims, int order = kXFast)
         const std::pair<int, int> g = dims(c->chains[ci]);
         for (int by = 0; by < g.second; by++) {
             for (int bx = 0; bx < g.first; bx++) {
-                v.push_back(BlockRef{(uint32_t)ci, (uint32_t)bx, (uint32_t)by, 0u});
+                v.push_back(BlockRef{(uint32_t)ci, (uint32_t)bx, (uint32_t)by, role});
             }
         }
     }
+}
+
+int upload_plan(sgpu_ctx *c, Plan &p, std::vector<BlockRef> &v, int order);
+
+// Block map over the chains selected by sel; dims(chain) gives the chain's
+// own grid (nx, ny).  Blocks are listed chain by chain, in the given order.
+template <class Sel, class Dims>
+int build_plan(sgpu_ctx *c, Plan &p, Sel sel, Dims dims, int order = kXFast)
+{
+    std::vector<BlockRef> v;
+    add_blocks(c, v, sel, dims);
+    return upload_plan(c, p, v, order);
+}
+
+int upload_plan(sgpu_ctx *c, Plan &p, std::vector<BlockRef> &v, int order)
+{
     if (order & kXcdRuns) {
         const size_t run = v.size() / kXcds;   // the last v.size() % kXcds blocks stay at the end
         std::vector<BlockRef> w(v);
@@ -347,6 +371,7 @@ mkern_t metro_kernel(int fmt, int K)
 
 int launch_z(sgpu_ctx *c, int mode)
 {
+    c->fold_fresh = false;   // new ancestry-count partials
     for (const KGroup &g : c->z_groups) {
         zkern_t k = z_kernel(g.fmt, g.K, mode, g.pf);
         if (!k) {
@@ -873,6 +898,7 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
     d.q_direct = d.n_lt <= kQDirectRows ? 1 : 0;
     d.q_blocks = (int)cdiv(N, kNT);
     d.l_blocks = (int)cdiv(L, kNT);
+    d.e_blocks = (int)cdiv(L, kNT / K);
     {
         const int n_ranges = (int)cdiv(L, std::min<size_t>(bt.metro_lpr, cdiv(L, 4) * 4));
         d.loci_per_range = (int)(cdiv(cdiv(L, n_ranges), 4) * 4);   // even, multiple of 4 loci
@@ -910,16 +936,18 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         (rc = dalloc(c, &d.likepart, (size_t)d.n_it * d.n_lt)) ||
         (rc = dalloc(c, &d.logq_part, cdiv(N, kNT / K) * (K + 1))) ||
         (rc = dalloc(c, &d.delta_part, (size_t)d.n_ranges * N)) ||
-        (rc = dalloc(c, &d.qprop, (size_t)N * K)) || (rc = dalloc(c, &d.fprop, (size_t)K)) ||
-        (rc = dalloc(c, &d.lam_part, (size_t)d.l_blocks * K)) ||
-        (rc = dalloc(c, &d.fst_part, (size_t)d.l_blocks * K)) ||
+        (rc = dalloc(c, &d.qprop, (size_t)N * K)) ||
+        (rc = dalloc(c, &d.fst_part, (size_t)std::max(d.l_blocks, d.e_blocks) * K)) ||
         (rc = dalloc(c, &d.QSum, (size_t)N * K)) || (rc = dalloc(c, &d.PSum, (size_t)L * K * A)) ||
         (rc = dalloc(c, &d.epsSum, (size_t)L * A)) || (rc = dalloc(c, &d.FstSum, K)) ||
         (rc = dalloc(c, &d.alphaSum, K)) || (rc = dalloc(c, &d.lambdaSum, K)) ||
         (rc = dalloc(c, &d.like3, 3))) {
         return rc;
     }
-    return build_p_slots(c, h, ds);
+    if ((rc = build_p_slots(c, h, ds))) {
+        return rc;
+    }
+    return dalloc(c, &d.lam_part, (size_t)std::max(d.l_blocks, d.p_blocks) * K);
 }
 
 int upload_descriptors(sgpu_ctx *c)
@@ -954,17 +982,43 @@ int build_plans(sgpu_ctx *c)
         (rc = build_plan(c, c->p_lam_part, [](const HostChain &h) { return h.m.inferlambda != 0; }, lb)) ||
         (rc = build_plan(c, c->p_lam_fin, [](const HostChain &h) { return h.m.inferlambda != 0; }, one)) ||
         (rc = build_plan(c, c->p_eps, [](const HostChain &h) { return h.m.freqscorr != 0; },
-                         [](const HostChain &h) { return std::make_pair((int)cdiv(h.m.L, kNT / h.m.K), 1); })) ||
+                         [](const HostChain &h) { return std::make_pair(h.d.e_blocks, 1); })) ||
         (rc = build_plan(c, c->p_fst_part, [](const HostChain &h) { return h.m.freqscorr != 0; },
                          [](const HostChain &h) {
                              return std::make_pair(h.d.l_blocks, h.m.onefst ? 1 : h.m.K);
                          })) ||
         (rc = build_plan(c, c->p_fst_fin, [](const HostChain &h) { return h.m.freqscorr != 0; }, one)) ||
         (rc = build_plan(c, c->p_collect, all_chains, one)) ||
+        (rc = build_plan(c, c->p_tail, [](const HostChain &h) { return h.m.freqscorr != 0; }, one)) ||
         (rc = build_plan(c, c->p_init_eps, all_chains, [](const HostChain &h) {
              return std::make_pair((int)cdiv(h.m.L, kNT / 64), 1);   // a wavefront per locus
          }))) {
         return rc;
+    }
+    {
+        // sweep launches mixing roles; the single blocks of a chain come first so
+        // that they start with the launch
+        auto gibbs_dims = [](const HostChain &h) {
+            return std::make_pair((int)cdiv(h.m.N, kNT / h.m.K), 1);
+        };
+        std::vector<BlockRef> v;
+        add_blocks(c, v, [](const HostChain &h) { return h.m.inferlambda != 0; }, one, kRoleLambda);
+        add_blocks(c, v, all_chains, gibbs_dims);
+        if ((rc = upload_plan(c, c->p_gibbs_sw, v, kXFast))) {
+            return rc;
+        }
+        v.clear();
+        add_blocks(c, v, all_chains, one, kRoleScalars);
+        add_blocks(c, v, [](const HostChain &h) { return h.m.freqscorr != 0; },
+                   [](const HostChain &h) { return std::make_pair(h.d.e_blocks, 1); });
+        add_blocks(c, v, [](const HostChain &h) { return h.d.q_direct == 0; },
+                   [](const HostChain &h) {
+                       return std::make_pair((int)cdiv((size_t)h.m.N * h.m.K, kNT), kRowSplit);
+                   },
+                   kRoleFold);
+        if ((rc = upload_plan(c, c->p_postz, v, kXFast))) {
+            return rc;
+        }
     }
     c->p_logq_chain.resize(c->chains.size());
     for (size_t ci = 0; ci < c->chains.size(); ci++) {
@@ -1365,6 +1419,9 @@ int sgpu_set_chain_state(sgpu_ctx *c, int32_t chain, int32_t which, const void *
         HIP_OK(hipMemcpyAsync(part, hp.data(), hp.size() * sizeof(uint16_t),
                               hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
+        if (!isA) {
+            c->fold_fresh = false;
+        }
         return SGPU_OK;
     }
     HIP_OK(hipMemcpyAsync(state_ptr(h.d, which), host, bytes, hipMemcpyHostToDevice, c->stream));
@@ -1385,21 +1442,39 @@ int sgpu_set_state(sgpu_ctx *c, int32_t which, const void *host, size_t bytes)
     return sgpu_set_chain_state(c, 0, which, host, bytes);
 }
 
-int sgpu_update_p(sgpu_ctx *c, uint32_t rep, int32_t collect)
+}  // extern "C"
+
+namespace {
+
+// UpdateP; with lam, the blocks also write the lambda log-sum partials.
+int update_p(sgpu_ctx *c, uint32_t rep, int32_t collect, int lam)
 {
     int rc = check_ready(c);
     if (rc || (rc = use_rep(c, rep))) {
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_P);
-    const uint32_t *r = c->repd;
-    if ((rc = launch(c, k_update_p, c->p_update_p, kNT, r, (int)collect))) {
+    return launch(c, k_update_p, c->p_update_p, kNT, (const uint32_t *)c->repd, (int)collect, lam);
+}
+
+// The fold of the ancestry-count partials the Gibbs update reads, unless nQfold
+// already holds the fold of the current partials.
+int ensure_fold(sgpu_ctx *c)
+{
+    if (c->fold_fresh) {
+        return SGPU_OK;
+    }
+    int rc = launch(c, k_fold_rows, c->p_fold, kNT);
+    if (rc) {
         return rc;
     }
+    c->fold_fresh = true;
     return SGPU_OK;
 }
 
-int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
+// UpdateQ over the (individual, pop) plan qp: p_gibbs, or p_gibbs_sw whose extra
+// blocks take each chain's lambda decision from UpdateP's partials.
+int update_q(sgpu_ctx *c, uint32_t rep, int32_t collect, const Plan &qp)
 {
     int rc = check_ready(c);
     if (rc || (rc = use_rep(c, rep))) {
@@ -1409,7 +1484,7 @@ int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
     const uint32_t *r = c->repd;
     const bool metro = c->metrofreq > 0 && rep % (uint32_t)c->metrofreq == 0;
     if (metro) {
-        if ((rc = launch(c, k_metro_propose, c->p_gibbs, kNT, r))) {   // (individual, pop) grid
+        if ((rc = launch(c, k_metro_propose, qp, kNT, r))) {   // (individual, pop) grid
             return rc;
         }
         for (const KGroup &g : c->metro_groups) {
@@ -1423,10 +1498,27 @@ int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
         }
         return launch(c, k_metro_accept, c->p_accept, kNT, r, (int)collect);
     }
-    if ((rc = launch(c, k_fold_rows, c->p_fold, kNT))) {
+    if ((rc = ensure_fold(c))) {
         return rc;
     }
-    return launch(c, k_update_q_gibbs, c->p_gibbs, kNT, r, (int)collect);
+    return launch(c, k_update_q_gibbs, qp, kNT, r, (int)collect);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sgpu_update_p(sgpu_ctx *c, uint32_t rep, int32_t collect)
+{
+    return update_p(c, rep, collect, 0);
+}
+
+int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
+{
+    if (!c) {
+        return fail(SGPU_ERR_ARG, "null context");
+    }
+    return update_q(c, rep, collect, c->p_gibbs);
 }
 
 int sgpu_update_z(sgpu_ctx *c, uint32_t rep, int32_t flags, int8_t *zout)
@@ -1493,7 +1585,8 @@ int sgpu_update_eps(sgpu_ctx *c, uint32_t rep, int32_t collect)
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_EPS);
-    return launch(c, k_update_eps, c->p_eps, kNT, (const uint32_t *)c->repd, (int)collect);
+    return launch(c, k_update_eps, c->p_eps, kNT, (const uint32_t *)c->repd,
+                  collect ? kEpsCollect : 0);
 }
 
 int sgpu_update_fst(sgpu_ctx *c, uint32_t rep)
@@ -1504,7 +1597,7 @@ int sgpu_update_fst(sgpu_ctx *c, uint32_t rep)
     }
     PhaseTimer timer(c, SGPU_PH_FST);
     const uint32_t *r = c->repd;
-    if ((rc = launch(c, k_fst_partial, c->p_fst_part, kNT, r, 0))) {
+    if ((rc = launch(c, k_fst_partial, c->p_fst_part, kNT, r))) {
         return rc;
     }
     return launch(c, k_fst_finalize, c->p_fst_fin, kNT, r);
@@ -1532,39 +1625,38 @@ int sgpu_sweep(sgpu_ctx *c, uint32_t rep)
     if (rc) {
         return rc;
     }
+    // The reference's phase order (structure.c:497-638) in five launches: UpdateP
+    // with the lambda partials; UpdateQ with the lambda decision (lambda is read
+    // next by the epsilon update, and neither UpdateQ nor UpdateZ touches it);
+    // UpdateZ; epsilon with the Fst partials, the count fold for the next sweep,
+    // and per chain alpha (it reads only Q and alpha), the lnP(D) sum and the
+    // scalar sums; the Fst decision with FstSum, which may close the sweep.
     const int collect = ((int64_t)rep + 1 > c->burnin) ? 1 : 0;
-    if ((rc = sgpu_update_p(c, rep, collect))) return rc;
-    if ((rc = sgpu_update_q(c, rep, collect))) return rc;
-    if ((rc = sgpu_update_z(c, rep, (collect && c->computeprob) ? SGPU_Z_LIKE : 0, nullptr)))
-        return rc;
-    if (c->any_lambda && (rc = sgpu_update_lambda(c, rep))) return rc;
-    if (c->any_freqscorr) {
-        if ((rc = sgpu_update_eps(c, rep, collect))) return rc;
-        PhaseTimer timer(c, SGPU_PH_FST);
-        if ((rc = launch(c, k_fst_partial, c->p_fst_part, kNT, (const uint32_t *)c->repd, 1))) {
+    const bool like = collect && c->computeprob;
+    if ((rc = update_p(c, rep, collect, c->any_lambda ? 1 : 0))) return rc;
+    if ((rc = update_q(c, rep, collect, c->p_gibbs_sw))) return rc;
+    if ((rc = sgpu_update_z(c, rep, like ? SGPU_Z_LIKE : 0, nullptr))) return rc;
+    const int ef = (collect ? kEpsCollect : 0) | (c->any_freqscorr ? kEpsFst : 0) |
+                   (c->any_alpha ? kEpsAlpha : 0) | ((like && c->like_ready) ? kEpsLike : 0);
+    if (c->any_freqscorr || c->p_fold.nblocks > 0 || (ef & (kEpsAlpha | kEpsCollect))) {
+        PhaseTimer timer(c, SGPU_PH_EPS);
+        if ((rc = launch(c, k_update_eps, c->p_postz, kNT, (const uint32_t *)c->repd, ef))) {
             return rc;
         }
+        c->fold_fresh = true;
     }
-    // alpha, the Fst decision and the data collection in one single-block-per-chain
-    // launch (alpha reads only Q and alpha, so it may close the sweep); in a
-    // captured sweep the same launch advances the device sweep index
-    int flags = (c->any_alpha ? kTailAlpha : 0) | (c->any_freqscorr ? kTailFst : 0);
-    if (collect) {
-        flags |= kTailCollect | ((c->computeprob && c->like_ready) ? kTailLike : 0);
-    }
+    // in a captured sweep the Fst launch advances the device sweep index
     c->tail_advances = false;
-    if (flags) {
-        if (c->capturing) {
-            flags |= kTailAdvance;
-        }
+    if (c->p_tail.nblocks > 0) {
+        const int tf = kTailFst | (collect ? kTailCollect : 0) | (c->capturing ? kTailAdvance : 0);
         PhaseTimer timer(c, SGPU_PH_COLLECT);
-        if ((rc = launch(c, k_sweep_tail, c->p_collect, kNT, c->repd, c->repd + 1, flags))) {
+        if ((rc = launch(c, k_sweep_tail, c->p_tail, kTailNT, c->repd, c->repd + 1, tf))) {
             return rc;
         }
-        c->tail_advances = (flags & kTailAdvance) != 0;
-        if (collect) {
-            c->nsum++;
-        }
+        c->tail_advances = (tf & kTailAdvance) != 0;
+    }
+    if (collect) {
+        c->nsum++;
     }
     return SGPU_OK;
 }
@@ -1585,7 +1677,9 @@ int sgpu_run_sweeps(sgpu_ctx *c, uint32_t rep0, uint32_t n)
     }
     // One graph per sweep type (Metropolis or Gibbs Q, burn-in or collecting),
     // captured on first use; each replay ends by advancing the device index.
-    if ((rc = use_rep(c, rep0))) {
+    // Every sweep leaves the count fold current for the next, so the graphs are
+    // captured (and replayed) with it current: fold now if it is not.
+    if ((rc = use_rep(c, rep0)) || (rc = ensure_fold(c))) {
         return rc;
     }
     for (uint32_t r = 0; r < n; r++) {

@@ -52,6 +52,7 @@ struct ChainDesc {
     int q_direct;                  // Gibbs Q sums the n_lt ancestry partials itself (no fold)
     int zt_loci, z_chunk;          // loci per UpdateZ block; byte path: loci staged in LDS at once
     int q_blocks, l_blocks;        // cdiv(N, 256), cdiv(L, 256)
+    int e_blocks;                  // epsilon-pass blocks: cdiv(L, 256 / K)
     int n_ranges, loci_per_range;  // Metropolis locus ranges
     int nhist, nloci_nonzero;      // allele-count histogram size, loci with alleles
     int p_blocks;                  // UpdateP blocks of this chain
@@ -73,8 +74,9 @@ struct ChainDesc {
     double *logq_part;             // [rows][K+1]
     double *delta_part;            // [n_ranges][N]
     double *qprop;                 // [N][K] Metropolis proposals of the current sweep
-    double *fprop;                 // [K] Fst proposals of the current sweep (epsilon pass)
-    double *lam_part, *fst_part;   // [l_blocks][K]
+    // lambda log-sum partials: [l_blocks][K] (lambda pass) or [p_blocks][K] (UpdateP);
+    // Fst log-ratio partials: [l_blocks][K] (Fst pass) or [e_blocks][K] (epsilon pass)
+    double *lam_part, *fst_part;
     double *QSum, *PSum, *epsSum, *FstSum, *alphaSum, *lambdaSum, *like3;
     int8_t *zout;                  // debug copy of Z [N][2][L] or null
     int32_t *adist;                // [N][K][numboxes] ANCESTDIST histogram, or null
@@ -103,10 +105,18 @@ __host__ __device__ inline T *g(T *p)
 }
 #endif
 
-// A block of a batched launch: its chain and its block coordinates within
-// that chain's own grid.
+// A block of a batched launch: its chain, its block coordinates within that
+// chain's own grid, and its role where one launch runs several passes.
 struct BlockRef {
-    uint32_t chain, bx, by, pad;
+    uint32_t chain, bx, by, role;
+};
+
+// Block roles of the launches that carry more than one pass.
+enum : uint32_t {
+    kRoleMain = 0,      // the launch's own pass
+    kRoleLambda = 1,    // Q-update launch: the lambda decision of the chain
+    kRoleFold = 2,      // epsilon launch: one block of the ancestry-count fold
+    kRoleScalars = 3    // epsilon launch: alpha, the lnP(D) sum and the scalar posterior sums
 };
 
 }  // namespace sgpu
