@@ -80,8 +80,9 @@ __device__ __forceinline__ double uo(uint32_t w) { return ((double)w + 0.5) * 0x
 // of 2 + f in [1.29, 2.42] is the hardware estimate refined by two Newton
 // steps.  Error about 1 ulp; 0 -> -inf, +inf -> +inf, negative or NaN -> NaN.
 // About 35 instructions instead of the library log's double-double
-// evaluation (over 100).
-__device__ __forceinline__ double log_d(double x)
+// evaluation (over 100).  log_pos is the same evaluation for finite x > 0
+// (subnormals included) without the special cases.
+__device__ __forceinline__ double log_pos(double x)
 {
     double m = __builtin_amdgcn_frexp_mant(x);
     int e = __builtin_amdgcn_frexp_exp(x);
@@ -109,7 +110,12 @@ __device__ __forceinline__ double log_d(double x)
     R = R * z;
     const double lm = fma(-s, f - R, f);
     const double de = (double)e;
-    const double r = fma(de, 0.6931471803691238, fma(de, 1.9082149292705877e-10, lm));
+    return fma(de, 0.6931471803691238, fma(de, 1.9082149292705877e-10, lm));
+}
+
+__device__ __forceinline__ double log_d(double x)
+{
+    const double r = log_pos(x);
     if (x > 0.0 && x < __builtin_inf()) {
         return r;
     }
@@ -246,15 +252,17 @@ __device__ __forceinline__ double lgamma_lanczos_terms(double z)
     s += c2 / den; den -= 1.0;
     s += c1 / den;
     s += 0.9999999999995183;
-    return log_d(s) + 0.9189385332046727 - (z + 6.5) + (z - 0.5) * log_d(z + 6.5);
+    return log_pos(s) + 0.9189385332046727 - (z + 6.5) + (z - 0.5) * log_pos(z + 6.5);
 }
 
 // ln Gamma(z), z > 0: Lanczos g = 7, n = 9 with the published coefficients,
 // the approximation the reference uses on the host (mymath.c:382-429).  The
 // eight partial fractions c_j / (z + j - 1) are accumulated as one fraction
-// n / d (n <- n e + c d, d <- d e), so the sum costs one division instead of
-// eight; the value agrees with the term-by-term sum to a few ulps.  Beyond
-// z = 1e30 (d would overflow) the term-by-term sum is used.
+// n / d (n <- n e + c d with one fused multiply-add, d <- d e), so the sum
+// costs one quotient instead of eight: the hardware reciprocal refined by two
+// Newton steps and one residual correction of n y.  The value agrees with the
+// term-by-term sum to a few ulps.  Beyond z = 1e30 (d would overflow) the
+// term-by-term sum is used.
 __device__ __forceinline__ double lgamma_lanczos(double z)
 {
     if (z > 1e30) {
@@ -266,12 +274,17 @@ __device__ __forceinline__ double lgamma_lanczos(double z)
     double n = 0.0, d = 1.0, e = z + 7.0;
 #pragma unroll
     for (int j = 7; j >= 0; j--) {   // z+7 down to z, as the host routine walks them
-        n = n * e + c[j] * d;
+        n = fma(n, e, c[j] * d);
         d = d * e;
         e -= 1.0;
     }
-    const double s = n / d + 0.9999999999995183;
-    return log_d(s) + 0.9189385332046727 - (z + 6.5) + (z - 0.5) * log_d(z + 6.5);
+    double y = __builtin_amdgcn_rcp(d);
+    y = fma(y, fma(-d, y, 1.0), y);
+    y = fma(y, fma(-d, y, 1.0), y);
+    double q = n * y;
+    q = fma(fma(-d, q, n), y, q);
+    const double s = q + 0.9999999999995183;
+    return log_pos(s) + 0.9189385332046727 - (z + 6.5) + (z - 0.5) * log_pos(z + 6.5);
 }
 
 // Wave-level (64 lanes) and block-level FP64 sums with a fixed reduction tree.

@@ -28,29 +28,35 @@ __device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint3
     const int bx = (int)br_.bx;                        \
     (void)bx
 
-// Sum over b < n of x[b * stride + off] by the 64 lanes of one wave: lane j
-// takes b = j, j + 64, ... into eight interleaved accumulators (eight loads in
-// flight), then a fixed tree over the accumulators and the lanes; the same
-// order whichever wave of a block runs it.  Valid in lane 0.
-__device__ __forceinline__ double wave_strided_sum(const SGPU_G double *x, int n, int stride,
-                                                   int off)
+// Sum of x[b * stride + off] over b = first, first + step, ... < n by one lane,
+// in eight interleaved accumulators (eight loads in flight) added in a fixed tree.
+__device__ __forceinline__ double lane_sum8(const SGPU_G double *x, int n, int stride, int off,
+                                            int first, int step)
 {
     double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    int b = (int)(threadIdx.x & 63);
-    for (; b + 7 * 64 < n; b += 8 * 64) {
+    int b = first;
+    for (; b + 7 * step < n; b += 8 * step) {
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            a[u] += x[(size_t)(b + u * 64) * stride + off];
+            a[u] += x[(size_t)(b + u * step) * stride + off];
         }
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-        if (b + u * 64 < n) {
-            a[u] += x[(size_t)(b + u * 64) * stride + off];
+        if (b + u * step < n) {
+            a[u] += x[(size_t)(b + u * step) * stride + off];
         }
     }
-    const double v = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-    return wave_sum(v);
+    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
+// Sum over b < n of x[b * stride + off] by the 64 lanes of one wave (lane j
+// takes b = j, j + 64, ...), then a fixed tree over the lanes; the same order
+// whichever wave of a block runs it.  Valid in lane 0.
+__device__ __forceinline__ double wave_strided_sum(const SGPU_G double *x, int n, int stride,
+                                                   int off)
+{
+    return wave_sum(lane_sum8(x, n, stride, off, (int)(threadIdx.x & 63), 64));
 }
 
 // Column sums of an [n][ncol] row-major table (ncol <= kMaxK + 1) by the 64
@@ -81,6 +87,56 @@ __device__ __forceinline__ void wave_col_sums(const SGPU_G double *x, int n, int
 }
 
 // ---------------------------------------------------------------------------
+// Lambda (UpdateLambda.c:8-102).  Partial sums (256 loci per block, loci with
+// more than one allele): sum_a log eps (F model), sum_k sum_a log P, or per pop
+// sum_a log P (pop-specific); row bx of lam_part.  The decision (one block per
+// chain) groups the normaliser terms by allele count (ahist = (value, number
+// of loci) pairs).
+// ---------------------------------------------------------------------------
+__device__ void lambda_partial_block(const ChainDesc &C, const ModelArgs &m, int bx)
+{
+    const SGPU_G int32_t *nall = g(C.nall);
+    const SGPU_G double *P = g(C.P);
+    const SGPU_G double *eps = g(C.eps);
+    SGPU_G double *part = g(C.lam_part);
+    __shared__ double s_red[kNT / 64];
+    const int l = bx * kNT + threadIdx.x;
+    const int na = (l < m.L) ? nall[l] : 0;
+    const int nsums = m.popspecificlambda ? m.K : 1;
+    for (int s = 0; s < nsums; s++) {
+        double v = 0.0;
+        if (na > 1) {
+            if (m.popspecificlambda) {
+                for (int aa = 0; aa < na; aa++) {
+                    v += log_d(P[((size_t)l * m.K + s) * m.A + aa]);
+                }
+            } else if (m.freqscorr) {
+                for (int aa = 0; aa < na; aa++) {
+                    v += log_d(eps[(size_t)l * m.A + aa]);
+                }
+            } else {
+                for (int k = 0; k < m.K; k++) {
+                    for (int aa = 0; aa < na; aa++) {
+                        v += log_d(P[((size_t)l * m.K + k) * m.A + aa]);
+                    }
+                }
+            }
+        }
+        const double t = block_sum<kNT>(v, s_red);
+        if (threadIdx.x == 0) {
+            part[(size_t)bx * nsums + s] = t;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kNT) void k_lambda_partial(const ChainDesc *__restrict__ cds,
+                                                         const BlockRef *__restrict__ map)
+{
+    SGPU_BLOCK_PROLOGUE();
+    lambda_partial_block(C, m, bx);
+}
+
+// ---------------------------------------------------------------------------
 // UpdateP (UpdateP.c:106-199): one thread per (locus, pop, allele), a block
 // holding whole (locus, pop) groups: nall[l] threads each from the chain's work
 // list, or A threads each (256 / A groups) on the dense grid.  Dirichlet
@@ -88,19 +144,19 @@ __device__ __forceinline__ void wave_col_sums(const SGPU_G double *x, int n, int
 // individual-tile partials, coalesced), one gamma draw per thread, then the
 // group's sum in allele order from LDS and LogRDirichletDisc's rule
 // (ran.c:1247-1272) including the sum < UNDERFLO uniform fallback.
-// Fused: PSum += P (output.c:673-677) when collecting; with `lam`, the block's
-// partial log sums of the lambda update (UpdateLambda.c:8-102) over loci with
-// more than one allele, one row of lam_part per block: sum of log eps (F
-// model: the epsilon this pass reads is the one lambda sees, as nothing writes
-// it in between), of log P of the new frequencies, or per pop of log P
-// (pop-specific lambda).
+// Fused: PSum += P (output.c:673-677) when collecting.  In a sweep the launch
+// also carries kRoleLamPart blocks: the lambda log sums of F-model chains read
+// only eps, which nothing writes between here and the lambda decision.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ cds,
                                                    const BlockRef *__restrict__ map,
-                                                   const uint32_t *__restrict__ repp, int collect,
-                                                   int lam)
+                                                   const uint32_t *__restrict__ repp, int collect)
 {
     SGPU_BLOCK_PROLOGUE();
+    if (br_.role == kRoleLamPart) {
+        lambda_partial_block(C, m, bx);
+        return;
+    }
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const SGPU_G int32_t *nall = g(C.nall);
     const SGPU_G uint16_t *nApart = g(C.nApart);
@@ -131,7 +187,7 @@ __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ 
     const int na = live ? nall[l] : 0;
     const bool act = aa < na;
     const size_t pos = ((size_t)l * m.K + k) * A + aa;
-    double x = 0.0, ev = 1.0;
+    double x = 0.0;
     if (act) {
         // the partials are independent loads: eight in flight, exact integer sums
         const size_t stride = (size_t)m.L * m.K * A;
@@ -158,45 +214,25 @@ __global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ 
         }
         const int n = (n0 + n1) + (n2 + n3);
         const double Fk = F[k];
-        ev = m.freqscorr ? eps[(size_t)l * A + aa] : 1.0;
-        const double par = m.freqscorr ? ev * (1.0 - Fk) / Fk + (double)n : lambda[k] + (double)n;
+        const double par = m.freqscorr ? eps[(size_t)l * A + aa] * (1.0 - Fk) / Fk + (double)n
+                                       : lambda[k] + (double)n;
         x = gamma_draw(par, m.seed, m.chain, rep, (uint32_t)l, ((uint32_t)k << 16) | (uint32_t)aa,
                        PH_P);
     }
     s_x[threadIdx.x] = x;
     __syncthreads();
-    double p = 1.0;
-    if (act) {
-        double sum = 0.0;
-        const int g0 = (int)threadIdx.x - aa;
-        for (int j = 0; j < na; j++) {   // allele order, as the host loop adds them
-            sum += s_x[g0 + j];
-        }
-        p = sum < kUnderflo ? 1.0 / (double)na : x / sum;
-        P[pos] = p;
-        if (collect) {
-            PSum[pos] += p;
-        }
+    if (!act) {
+        return;
     }
-    if (lam && m.inferlambda) {   // block-uniform
-        __shared__ double s_red[kNT / 64];
-        SGPU_G double *part = g(C.lam_part);
-        const bool use = act && na > 1;
-        if (m.popspecificlambda) {
-            const double v = use ? log_d(p) : 0.0;
-            for (int s = 0; s < m.K; s++) {
-                const double t = block_sum<kNT>(k == s ? v : 0.0, s_red);
-                if (threadIdx.x == 0) {
-                    part[(size_t)bx * m.K + s] = t;
-                }
-            }
-        } else {
-            const double v = !use ? 0.0 : (m.freqscorr ? (k == 0 ? log_d(ev) : 0.0) : log_d(p));
-            const double t = block_sum<kNT>(v, s_red);
-            if (threadIdx.x == 0) {
-                part[bx] = t;
-            }
-        }
+    double sum = 0.0;
+    const int g0 = (int)threadIdx.x - aa;
+    for (int j = 0; j < na; j++) {   // allele order, as the host loop adds them
+        sum += s_x[g0 + j];
+    }
+    const double p = sum < kUnderflo ? 1.0 / (double)na : x / sum;
+    P[pos] = p;
+    if (collect) {
+        PSum[pos] += p;
     }
 }
 
@@ -327,8 +363,12 @@ __global__ __launch_bounds__(kNT) void k_update_q_gibbs(const ChainDesc *__restr
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
-    if (br_.role == kRoleLambda) {   // the chain's lambda decision, from UpdateP's partials
-        lambda_final_block(C, m, rep, C.p_blocks);
+    if (br_.role == kRoleLambda) {   // the chain's lambda decision (F model)
+        lambda_final_block(C, m, rep, C.l_blocks);
+        return;
+    }
+    if (br_.role == kRoleLamPart) {  // lambda log sums of the new P (no F model)
+        lambda_partial_block(C, m, bx);
         return;
     }
     const SGPU_G double *alpha = g(C.alpha);
@@ -463,8 +503,12 @@ __global__ __launch_bounds__(kNT) void k_metro_propose(const ChainDesc *__restri
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
-    if (br_.role == kRoleLambda) {   // the chain's lambda decision, from UpdateP's partials
-        lambda_final_block(C, m, rep, C.p_blocks);
+    if (br_.role == kRoleLambda) {   // the chain's lambda decision (F model)
+        lambda_final_block(C, m, rep, C.l_blocks);
+        return;
+    }
+    if (br_.role == kRoleLamPart) {  // lambda log sums of the new P (no F model)
+        lambda_partial_block(C, m, bx);
         return;
     }
     __shared__ double s_x[kNT];
@@ -794,52 +838,10 @@ __global__ __launch_bounds__(kNT) void k_update_alpha(const ChainDesc *__restric
     }
 }
 
-// ---------------------------------------------------------------------------
-// Lambda (UpdateLambda.c:8-102).  Pass 1: per-locus log sums (loci with more
-// than one allele): sum_a log eps (F model), sum_k sum_a log P, or per pop
-// sum_a log P (pop-specific).  Pass 2 (one block per chain): the normaliser
-// terms are grouped by allele count (ahist = (value, number of loci) pairs).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNT) void k_lambda_partial(const ChainDesc *__restrict__ cds,
-                                                         const BlockRef *__restrict__ map)
-{
-    SGPU_BLOCK_PROLOGUE();
-    const SGPU_G int32_t *nall = g(C.nall);
-    const SGPU_G double *P = g(C.P);
-    const SGPU_G double *eps = g(C.eps);
-    SGPU_G double *part = g(C.lam_part);
-    __shared__ double s_red[kNT / 64];
-    const int l = bx * kNT + threadIdx.x;
-    const int na = (l < m.L) ? nall[l] : 0;
-    const int nsums = m.popspecificlambda ? m.K : 1;
-    for (int s = 0; s < nsums; s++) {
-        double v = 0.0;
-        if (na > 1) {
-            if (m.popspecificlambda) {
-                for (int aa = 0; aa < na; aa++) {
-                    v += log_d(P[((size_t)l * m.K + s) * m.A + aa]);
-                }
-            } else if (m.freqscorr) {
-                for (int aa = 0; aa < na; aa++) {
-                    v += log_d(eps[(size_t)l * m.A + aa]);
-                }
-            } else {
-                for (int k = 0; k < m.K; k++) {
-                    for (int aa = 0; aa < na; aa++) {
-                        v += log_d(P[((size_t)l * m.K + k) * m.A + aa]);
-                    }
-                }
-            }
-        }
-        const double t = block_sum<kNT>(v, s_red);
-        if (threadIdx.x == 0) {
-            part[(size_t)bx * nsums + s] = t;
-        }
-    }
-}
-
 // The lambda decision of one chain by a whole block of kNT threads: sums the
-// nparts rows of partials, then thread 0 runs the proposal(s).
+// nparts rows of partials (eight loads in flight per thread), then per proposal
+// the normaliser terms of the allele-count groups in parallel (thread h takes
+// groups h, h + kNT, ...) and thread 0 adds them in thread order and decides.
 __device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint32_t rep,
                                    int nparts)
 {
@@ -849,19 +851,14 @@ __device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint3
     const int nhist = C.nhist;
     __shared__ double s_red[kNT / 64];
     __shared__ double s_stat[kMaxK];
+    __shared__ double s_t[kNT];
     const int nsums = m.popspecificlambda ? m.K : 1;
     for (int s = 0; s < nsums; s++) {
-        double v = 0.0;
-        for (int b = threadIdx.x; b < nparts; b += kNT) {
-            v += part[(size_t)b * nsums + s];
-        }
-        const double t = block_sum<kNT>(v, s_red);
+        const double t = block_sum<kNT>(
+            lane_sum8(part, nparts, nsums, s, (int)threadIdx.x, kNT), s_red);
         if (threadIdx.x == 0) {
             s_stat[s] = t;
         }
-    }
-    if (threadIdx.x != 0) {
-        return;
     }
     const double stoppop = m.popspecificlambda ? 1.0 : (m.freqscorr ? 1.0 : (double)m.K);
     for (int s = 0; s < nsums; s++) {
@@ -869,17 +866,26 @@ __device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint3
         const double z = box_muller(w.x, w.y);
         const double old = lambda[s];
         const double nw = old + kLambdaPropSd * z;
-        if ((nw > 0.0) && (nw < kLambdaMax)) {
+        if (!((nw > 0.0) && (nw < kLambdaMax))) {   // block-uniform
+            continue;
+        }
+        double loc = 0.0;
+        for (int h = (int)threadIdx.x; h < nhist; h += kNT) {
+            const double na = (double)ahist[2 * h];
+            const double cnt = (double)ahist[2 * h + 1];
+            double t = 0.0;
+            t += stoppop * lgamma_lanczos(na * nw);
+            t -= stoppop * lgamma_lanczos(na * old);
+            t += stoppop * na * lgamma_lanczos(old);
+            t -= stoppop * na * lgamma_lanczos(nw);
+            loc += cnt * t;
+        }
+        s_t[threadIdx.x] = loc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
             double sum = 0.0;
-            for (int h = 0; h < nhist; h++) {
-                const double na = (double)ahist[2 * h];
-                const double cnt = (double)ahist[2 * h + 1];
-                double t = 0.0;
-                t += stoppop * lgamma_lanczos(na * nw);
-                t -= stoppop * lgamma_lanczos(na * old);
-                t += stoppop * na * lgamma_lanczos(old);
-                t -= stoppop * na * lgamma_lanczos(nw);
-                sum += cnt * t;
+            for (int h = 0; h < nhist && h < kNT; h++) {
+                sum += s_t[h];
             }
             sum += (nw - old) * s_stat[s];
             if (uo(w.z) < exp(sum)) {
@@ -892,6 +898,7 @@ __device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint3
                 }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -912,28 +919,6 @@ __device__ __forceinline__ double fst_proposal(const ModelArgs &m, double oldf, 
         *uw = w.z;
     }
     return oldf + m.fpriorsd * box_muller(w.x, w.y);
-}
-
-// Sum of x[b * stride + off] over b = first, first + step, ... < n by one lane,
-// in eight interleaved accumulators (eight loads in flight) added in a fixed tree.
-__device__ __forceinline__ double lane_sum8(const SGPU_G double *x, int n, int stride, int off,
-                                            int first, int step)
-{
-    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    int b = first;
-    for (; b + 7 * step < n; b += 8 * step) {
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            a[u] += x[(size_t)(b + u * step) * stride + off];
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-        if (b + u * step < n) {
-            a[u] += x[(size_t)(b + u * step) * stride + off];
-        }
-    }
-    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
 }
 
 // Column sums of an [n][ncol] row-major table (ncol <= kMaxK + 1) by a block of
@@ -989,16 +974,20 @@ __device__ __forceinline__ double like_share(const ChainDesc &C, int s, int ns)
 // Flags of the epsilon launch.
 constexpr int kEpsCollect = 1, kEpsFst = 2, kEpsAlpha = 4, kEpsLike = 8;
 
-// The kRoleScalars block of a chain in the epsilon launch (kNT threads): wave 0
-// runs the alpha update (it reads only Q, whose update closed before UpdateZ,
-// and alpha), waves 1-3 sum thirds of the lnP(D) tile partials; then wave 0
-// adds lambda and alpha (both final for the sweep: the lambda decision ran in
-// the Q-update launch) and the lnP accumulators into the posterior sums
-// (output.c:50-101, UpdateSums :654-707).  F and its sum follow in the tail.
+// The kRoleScalars block of a chain in the epsilon launch (kNT threads): the
+// lambda decision of a chain without the F model (nothing reads lambda between
+// here and the next UpdateP); then wave 0 runs the alpha update (it reads only
+// Q, whose update closed before UpdateZ, and alpha), waves 1-3 sum thirds of
+// the lnP(D) tile partials; then wave 0 adds lambda and alpha (both final for
+// the sweep) and the lnP accumulators into the posterior sums (output.c:50-101,
+// UpdateSums :654-707).  F and its sum follow in the tail.
 __device__ void scalars_block(const ChainDesc &C, const ModelArgs &m, uint32_t rep, int flags)
 {
     __shared__ double s_like[3];
     const int wid = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    if (C.lam_mode == kLamP) {   // lambda, from the Q-update launch's log sums
+        lambda_final_block(C, m, rep, C.l_blocks);
+    }
     if (wid == 0 && (flags & kEpsAlpha) && m.inferalpha) {
         alpha_wave(C, m, rep);
     }

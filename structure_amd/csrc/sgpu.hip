@@ -127,10 +127,12 @@ struct sgpu_ctx {
     std::vector<KGroup> z_groups, metro_groups;
     Plan p_update_p, p_fold, p_gibbs, p_logq, p_accept, p_alpha, p_lam_part,
         p_lam_fin, p_eps, p_fst_part, p_fst_fin, p_collect, p_init_eps;
-    // the fused launches of a sweep: the Q-update launch with each chain's lambda
-    // decision block; the epsilon launch with the scalar blocks, epsilon + Fst
-    // blocks and count-fold blocks; the Fst decision (F-model chains)
-    Plan p_gibbs_sw, p_postz, p_tail;
+    // the fused launches of a sweep: UpdateP with the lambda log-sum blocks of
+    // F-model chains; the Q update with their lambda decision blocks and the
+    // log-sum blocks of the other chains; the epsilon launch with the scalar
+    // blocks, epsilon + Fst blocks and count-fold blocks; the Fst decision
+    // (F-model chains); separate lambda launches for pop-specific lambda with F
+    Plan p_update_p_sw, p_gibbs_sw, p_postz, p_tail, p_lam_part_x, p_lam_fin_x;
     // nQfold holds the fold of the current nQpart (false after UpdateZ or a host
     // write of the ancestry counts, until a fold runs)
     bool fold_fresh = false;
@@ -944,10 +946,13 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         (rc = dalloc(c, &d.like3, 3))) {
         return rc;
     }
+    d.lam_mode = !m.inferlambda ? kLamNone
+                 : !m.freqscorr    ? kLamP
+                 : (m.popspecificlambda ? kLamSplit : kLamEps);
     if ((rc = build_p_slots(c, h, ds))) {
         return rc;
     }
-    return dalloc(c, &d.lam_part, (size_t)std::max(d.l_blocks, d.p_blocks) * K);
+    return dalloc(c, &d.lam_part, (size_t)d.l_blocks * K);
 }
 
 int upload_descriptors(sgpu_ctx *c)
@@ -1001,10 +1006,23 @@ int build_plans(sgpu_ctx *c)
         auto gibbs_dims = [](const HostChain &h) {
             return std::make_pair((int)cdiv(h.m.N, kNT / h.m.K), 1);
         };
+        auto mode = [](int lm) { return [lm](const HostChain &h) { return h.d.lam_mode == lm; }; };
         std::vector<BlockRef> v;
-        add_blocks(c, v, [](const HostChain &h) { return h.m.inferlambda != 0; }, one, kRoleLambda);
+        add_blocks(c, v, all_chains,
+                   [](const HostChain &h) { return std::make_pair(h.d.p_blocks, 1); });
+        add_blocks(c, v, mode(kLamEps), lb, kRoleLamPart);
+        if ((rc = upload_plan(c, c->p_update_p_sw, v, kXFast))) {
+            return rc;
+        }
+        v.clear();
+        add_blocks(c, v, mode(kLamEps), one, kRoleLambda);
         add_blocks(c, v, all_chains, gibbs_dims);
+        add_blocks(c, v, mode(kLamP), lb, kRoleLamPart);
         if ((rc = upload_plan(c, c->p_gibbs_sw, v, kXFast))) {
+            return rc;
+        }
+        if ((rc = build_plan(c, c->p_lam_part_x, mode(kLamSplit), lb)) ||
+            (rc = build_plan(c, c->p_lam_fin_x, mode(kLamSplit), one))) {
             return rc;
         }
         v.clear();
@@ -1446,15 +1464,15 @@ int sgpu_set_state(sgpu_ctx *c, int32_t which, const void *host, size_t bytes)
 
 namespace {
 
-// UpdateP; with lam, the blocks also write the lambda log-sum partials.
-int update_p(sgpu_ctx *c, uint32_t rep, int32_t collect, int lam)
+// UpdateP over the plan pp: p_update_p, or p_update_p_sw with lambda log-sum blocks.
+int update_p(sgpu_ctx *c, uint32_t rep, int32_t collect, const Plan &pp)
 {
     int rc = check_ready(c);
     if (rc || (rc = use_rep(c, rep))) {
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_P);
-    return launch(c, k_update_p, c->p_update_p, kNT, (const uint32_t *)c->repd, (int)collect, lam);
+    return launch(c, k_update_p, pp, kNT, (const uint32_t *)c->repd, (int)collect);
 }
 
 // The fold of the ancestry-count partials the Gibbs update reads, unless nQfold
@@ -1472,8 +1490,8 @@ int ensure_fold(sgpu_ctx *c)
     return SGPU_OK;
 }
 
-// UpdateQ over the (individual, pop) plan qp: p_gibbs, or p_gibbs_sw whose extra
-// blocks take each chain's lambda decision from UpdateP's partials.
+// UpdateQ over the (individual, pop) plan qp: p_gibbs, or p_gibbs_sw with the
+// lambda blocks of a sweep.
 int update_q(sgpu_ctx *c, uint32_t rep, int32_t collect, const Plan &qp)
 {
     int rc = check_ready(c);
@@ -1510,7 +1528,10 @@ extern "C" {
 
 int sgpu_update_p(sgpu_ctx *c, uint32_t rep, int32_t collect)
 {
-    return update_p(c, rep, collect, 0);
+    if (!c) {
+        return fail(SGPU_ERR_ARG, "null context");
+    }
+    return update_p(c, rep, collect, c->p_update_p);
 }
 
 int sgpu_update_q(sgpu_ctx *c, uint32_t rep, int32_t collect)
@@ -1625,17 +1646,26 @@ int sgpu_sweep(sgpu_ctx *c, uint32_t rep)
     if (rc) {
         return rc;
     }
-    // The reference's phase order (structure.c:497-638) in five launches: UpdateP
-    // with the lambda partials; UpdateQ with the lambda decision (lambda is read
-    // next by the epsilon update, and neither UpdateQ nor UpdateZ touches it);
-    // UpdateZ; epsilon with the Fst partials, the count fold for the next sweep,
-    // and per chain alpha (it reads only Q and alpha), the lnP(D) sum and the
-    // scalar sums; the Fst decision with FstSum, which may close the sweep.
+    // The reference's phase order (structure.c:497-638) in five launches.  The
+    // lambda update moves, as nothing between reads or writes what it reads and
+    // writes: F-model chains sum log eps beside UpdateP and decide beside UpdateQ
+    // (lambda is read next by the epsilon update); other chains sum log P beside
+    // UpdateQ and decide in the epsilon launch (lambda is read next by UpdateP).
+    // Then UpdateZ; epsilon with the Fst partials, the count fold for the next
+    // sweep, and per chain alpha (it reads only Q and alpha), the lnP(D) sum and
+    // the scalar sums; the Fst decision with FstSum, which may close the sweep.
     const int collect = ((int64_t)rep + 1 > c->burnin) ? 1 : 0;
     const bool like = collect && c->computeprob;
-    if ((rc = update_p(c, rep, collect, c->any_lambda ? 1 : 0))) return rc;
+    if ((rc = update_p(c, rep, collect, c->p_update_p_sw))) return rc;
     if ((rc = update_q(c, rep, collect, c->p_gibbs_sw))) return rc;
     if ((rc = sgpu_update_z(c, rep, like ? SGPU_Z_LIKE : 0, nullptr))) return rc;
+    if (c->p_lam_fin_x.nblocks > 0) {   // pop-specific lambda with the F model
+        PhaseTimer timer(c, SGPU_PH_LAMBDA);
+        if ((rc = launch(c, k_lambda_partial, c->p_lam_part_x, kNT)) ||
+            (rc = launch(c, k_lambda_finalize, c->p_lam_fin_x, kNT, (const uint32_t *)c->repd))) {
+            return rc;
+        }
+    }
     const int ef = (collect ? kEpsCollect : 0) | (c->any_freqscorr ? kEpsFst : 0) |
                    (c->any_alpha ? kEpsAlpha : 0) | ((like && c->like_ready) ? kEpsLike : 0);
     if (c->any_freqscorr || c->p_fold.nblocks > 0 || (ef & (kEpsAlpha | kEpsCollect))) {

@@ -56,6 +56,7 @@ struct ChainDesc {
     int n_ranges, loci_per_range;  // Metropolis locus ranges
     int nhist, nloci_nonzero;      // allele-count histogram size, loci with alleles
     int p_blocks;                  // UpdateP blocks of this chain
+    int lam_mode;                  // kLamNone .. kLamSplit
     // UpdateP work list [p_blocks][256]: (l << 13 | k << 8 | a) for a < nall[l],
     // each (locus, pop) group whole within one block, ~0 = idle thread; null:
     // the dense (locus, pop, allele < A) grid
@@ -74,8 +75,8 @@ struct ChainDesc {
     double *logq_part;             // [rows][K+1]
     double *delta_part;            // [n_ranges][N]
     double *qprop;                 // [N][K] Metropolis proposals of the current sweep
-    // lambda log-sum partials: [l_blocks][K] (lambda pass) or [p_blocks][K] (UpdateP);
-    // Fst log-ratio partials: [l_blocks][K] (Fst pass) or [e_blocks][K] (epsilon pass)
+    // lambda log-sum partials [l_blocks][K]; Fst log-ratio partials: [l_blocks][K]
+    // (Fst pass) or [e_blocks][K] (epsilon pass)
     double *lam_part, *fst_part;
     double *QSum, *PSum, *epsSum, *FstSum, *alphaSum, *lambdaSum, *like3;
     int8_t *zout;                  // debug copy of Z [N][2][L] or null
@@ -116,7 +117,18 @@ enum : uint32_t {
     kRoleMain = 0,      // the launch's own pass
     kRoleLambda = 1,    // Q-update launch: the lambda decision of the chain
     kRoleFold = 2,      // epsilon launch: one block of the ancestry-count fold
-    kRoleScalars = 3    // epsilon launch: alpha, the lnP(D) sum and the scalar posterior sums
+    kRoleScalars = 3,   // epsilon launch: alpha, the lnP(D) sum and the scalar posterior sums
+    kRoleLamPart = 4    // UpdateP or Q-update launch: 256 loci of the lambda log sums
+};
+
+// Where a sweep runs a chain's lambda update (UpdateLambda.c:8-102): its log
+// sums read eps (F model) or the new P, and its decision must precede the
+// epsilon update, which reads lambda.
+enum : int {
+    kLamNone = 0,    // lambda not inferred
+    kLamEps = 1,     // F model: log sums in the UpdateP launch, decision in the Q-update launch
+    kLamP = 2,       // no F model: log sums in the Q-update launch, decision in the epsilon launch
+    kLamSplit = 3    // F model with pop-specific lambda (log sums of the new P): own launches
 };
 
 }  // namespace sgpu
