@@ -720,6 +720,10 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_byte(const ChainDesc *__res
     }
 }
 
+// Individuals per Metropolis acceptance block: each takes four waves, wave h
+// summing the ranges r = h, h + 4, ... of its individuals' log-ratio partials.
+constexpr int kAcceptInds = 64;
+
 __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restrict__ cds,
                                                        const BlockRef *__restrict__ map,
                                                        const uint32_t *__restrict__ repp,
@@ -729,25 +733,43 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restric
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     const SGPU_G double *delta_part = g(C.delta_part);
     SGPU_G double *Q = g(C.Q);
-    const int n_ranges = C.n_ranges;
     __shared__ double s_red[kNT / 64];
-    const int i = bx * kNT + threadIdx.x;
-    const bool active = i < m.N;
+    __shared__ double s_d[kNT];
+    const int li = (int)(threadIdx.x & 63), h = (int)(threadIdx.x >> 6);
+    const int i = bx * kAcceptInds + li;
+    const bool inds = i < m.N;
+    // a quarter of the range partials per wave, sixteen loads in flight per lane
+    double a[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        a[u] = 0.0;
+    }
+    if (inds) {
+        int r = h;
+        for (; r + 15 * 4 < C.n_ranges; r += 16 * 4) {
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                a[u] += delta_part[(size_t)(r + 4 * u) * m.N + i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            if (r + 4 * u < C.n_ranges) {
+                a[u] += delta_part[(size_t)(r + 4 * u) * m.N + i];
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        a[u] += a[u + 8];
+    }
+    s_d[threadIdx.x] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    __syncthreads();
+    // wave 0: one individual per lane, the quarters added in wave order
+    const bool active = h == 0 && inds;
     double q[kMaxK];
     if (active) {
-        // the range partials in four interleaved sums (loads in flight), fixed order
-        double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-        int r = 0;
-        for (; r + 4 <= n_ranges; r += 4) {
-            d0 += delta_part[(size_t)r * m.N + i];
-            d1 += delta_part[(size_t)(r + 1) * m.N + i];
-            d2 += delta_part[(size_t)(r + 2) * m.N + i];
-            d3 += delta_part[(size_t)(r + 3) * m.N + i];
-        }
-        for (; r < n_ranges; r++) {
-            d0 += delta_part[(size_t)r * m.N + i];
-        }
-        const double delta = (d0 + d1) + (d2 + d3);
+        const double delta = (s_d[li] + s_d[64 + li]) + (s_d[128 + li] + s_d[192 + li]);
         const u32x4 w = philox4x32_10(rep, (uint32_t)i, 0xffffffffu, PH_QMETRO << 24, m.seed,
                                       m.chain);
         const bool acc = uo(w.x) < exp(delta);
@@ -756,8 +778,8 @@ __global__ __launch_bounds__(kNT) void k_metro_accept(const ChainDesc *__restric
             q[k] = src[(size_t)i * m.K + k];
         }
     }
-    finish_q<kNT>(m, bx, C.q_blocks, i, active, q, Q, g(C.QSum), collect, g(C.logq_part),
-                  g(C.logq_rows), g(C.adist), s_red);
+    finish_q<kNT>(m, bx, (m.N + kAcceptInds - 1) / kAcceptInds, i, active, q, Q, g(C.QSum),
+                  collect, g(C.logq_part), g(C.logq_rows), g(C.adist), s_red);
 }
 
 // ---------------------------------------------------------------------------
@@ -1050,10 +1072,9 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
     const SGPU_G double *lambda = g(C.lambda);
     SGPU_G double *eps = g(C.eps);
     SGPU_G double *epsSum = g(C.epsSum);
-    __shared__ double s_term[6][kNT];   // the six terms of each (locus, pop), summed in order below
+    __shared__ double s_t[kNT];         // each (locus, pop)'s part of the move's log-ratio
     __shared__ double s_f[kNT];         // Fst terms of each (locus, pop)
     __shared__ double s_newf[kMaxK];    // this sweep's Fst proposals
-    __shared__ int s_acc[kNT];          // per locus of the block: the move was accepted
     const int K = m.K;
     const int lpb = kNT / K;                       // loci per block
     const int j = threadIdx.x / K, k = threadIdx.x % K;
@@ -1082,7 +1103,7 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
         inrange = (e1 + d < 1.0) && (e2 - d > 0.0);
     }
     const double frac = (1.0 - F[k]) / F[k];
-    double lg0 = 0.0, lg1 = 0.0, lg2 = 0.0, lg3 = 0.0, lp1 = 0.0, lp2 = 0.0;
+    double lg0 = 0.0, lg1 = 0.0, lg2 = 0.0, lg3 = 0.0, lp1 = 0.0, lp2 = 0.0, tk = 0.0;
     if (inrange) {
         lg0 = lgamma_lanczos(frac * e1);
         lg1 = lgamma_lanczos(frac * e2);
@@ -1090,50 +1111,43 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
         lg3 = lgamma_lanczos(frac * (e2 - d));
         lp1 = log_d(P[((size_t)l * K + k) * m.A + a1]);
         lp2 = log_d(P[((size_t)l * K + k) * m.A + a2]);
-        s_term[0][threadIdx.x] = lg0;
-        s_term[1][threadIdx.x] = lg1;
-        s_term[2][threadIdx.x] = lg2;
-        s_term[3][threadIdx.x] = lg3;
-        s_term[4][threadIdx.x] = frac * d * lp1;
-        s_term[5][threadIdx.x] = frac * d * lp2;
+        // pop k's six terms in the reference's order within a pop
+        tk = ((((lg0 + lg1) - lg2) - lg3) + frac * d * lp1) - frac * d * lp2;
     }
+    s_t[threadIdx.x] = tk;
     __syncthreads();
-    if (live && k == 0) {
-        int acc = 0;
-        if (inrange) {
-            double sum = 0.0;
-            for (int kk = 0; kk < K; kk++) {   // the reference's accumulation order
-                const int t = threadIdx.x + kk;
-                sum += s_term[0][t];
-                sum += s_term[1][t];
-                sum -= s_term[2][t];
-                sum -= s_term[3][t];
-                sum += s_term[4][t];
-                sum -= s_term[5][t];
-            }
-            const double lam = lambda[0];
-            if (lam != 1.0) {
-                // log(pow(x, lam - 1)) of UpdateEpsilon.c:117-118 as (lam - 1) log x
-                // (equal but for rounding, and no pow)
-                sum += ((double)lam - 1.0) * log_d((e1 + d) * (e2 - d) / (e1) / (e2));
-            }
-            if (uo(w.w) < exp(sum)) {
-                e[a1] = e1 + d;
-                e[a2] = e2 - d;
-                acc = 1;
-            }
+    // every thread of the locus adds the K pop parts in pop order and decides:
+    // the same sum and the same uniform, so the same decision in each
+    bool acc = false;
+    if (inrange) {
+        const int t0 = (int)threadIdx.x - k;
+        double sum = 0.0;
+        for (int kk = 0; kk < K; kk++) {
+            sum += s_t[t0 + kk];
         }
-        s_acc[j] = acc;
+        const double lam = lambda[0];
+        if (lam != 1.0) {
+            // log(pow(x, lam - 1)) of UpdateEpsilon.c:117-118 as (lam - 1) log x
+            // (equal but for rounding, and no pow)
+            sum += ((double)lam - 1.0) * log_d((e1 + d) * (e2 - d) / (e1) / (e2));
+        }
+        acc = uo(w.w) < exp(sum);
+    }
+    if (live && k == 0) {
+        if (acc) {
+            e[a1] = e1 + d;
+            e[a2] = e2 - d;
+        }
         if (flags & kEpsCollect) {
             for (int aa = 0; aa < na; aa++) {
-                epsSum[(size_t)l * m.A + aa] += e[aa];
+                const double ev = (acc && aa == a1) ? e1 + d : ((acc && aa == a2) ? e2 - d : e[aa]);
+                epsSum[(size_t)l * m.A + aa] += ev;
             }
         }
     }
     if (!fst) {
         return;
     }
-    __syncthreads();
     double v = 0.0;
     const int grp = m.onefst ? 0 : k;
     const double newf = s_newf[grp];
@@ -1141,7 +1155,6 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
         const double oldf = F[grp];
         const double nfr = (1.0 - newf) / newf;
         const double ofr = (1.0 - oldf) / oldf;
-        const bool acc = inrange && s_acc[j] != 0;
         const bool reuse = inrange && ofr == frac;   // the move's lgamma arguments are ours
         const SGPU_G double *pl = P + ((size_t)l * K + k) * m.A;
         for (int aa = 0; aa < na; aa++) {

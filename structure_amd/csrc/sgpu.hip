@@ -936,7 +936,8 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
         (rc = dalloc(c, &d.nQfold, (size_t)kRowSplit * N * K)) ||
         (rc = dalloc(c, &d.logq_rows, 1)) ||
         (rc = dalloc(c, &d.likepart, (size_t)d.n_it * d.n_lt)) ||
-        (rc = dalloc(c, &d.logq_part, cdiv(N, kNT / K) * (K + 1))) ||
+        (rc = dalloc(c, &d.logq_part,
+                     std::max(cdiv(N, kNT / K), cdiv(N, kAcceptInds)) * (K + 1))) ||
         (rc = dalloc(c, &d.delta_part, (size_t)d.n_ranges * N)) ||
         (rc = dalloc(c, &d.qprop, (size_t)N * K)) ||
         (rc = dalloc(c, &d.fst_part, (size_t)std::max(d.l_blocks, d.e_blocks) * K)) ||
@@ -982,7 +983,10 @@ int build_plans(sgpu_ctx *c)
         (rc = build_plan(c, c->p_gibbs, all_chains,
                          [](const HostChain &h) { return std::make_pair((int)cdiv(h.m.N, kNT / h.m.K), 1); })) ||
         (rc = build_plan(c, c->p_logq, all_chains, qb)) ||
-        (rc = build_plan(c, c->p_accept, all_chains, qb)) ||
+        (rc = build_plan(c, c->p_accept, all_chains,
+                         [](const HostChain &h) {
+                             return std::make_pair((int)cdiv(h.m.N, kAcceptInds), 1);
+                         })) ||
         (rc = build_plan(c, c->p_alpha, [](const HostChain &h) { return h.m.inferalpha != 0; }, one)) ||
         (rc = build_plan(c, c->p_lam_part, [](const HostChain &h) { return h.m.inferlambda != 0; }, lb)) ||
         (rc = build_plan(c, c->p_lam_fin, [](const HostChain &h) { return h.m.inferlambda != 0; }, one)) ||
