@@ -101,11 +101,20 @@ def test_init_state_matches_oracle(kind):
         assert np.array_equal(gch.get(sa.ALPHA), och.alpha)
 
 
+@pytest.mark.parametrize("kind", ["microsat", "biallelic"])
 @pytest.mark.parametrize("freqscorr", [1, 0])
-def test_update_p_matches_oracle(freqscorr):
+def test_update_p_matches_oracle(freqscorr, kind):
     rng = np.random.default_rng(7)
-    d = sa.synth_microsat(30, 200, amin=1, amax=12, Ktrue=3, seed=5)
-    geno, nall, A = d["geno"], d["nall"], d["A"]
+    if kind == "microsat":
+        d = sa.synth_microsat(30, 200, amin=1, amax=12, Ktrue=3, seed=5)
+        geno, nall, A = d["geno"], d["nall"], d["A"]
+    else:   # A = 2 (one thread per locus and pop), with monomorphic and empty loci
+        geno, nall = random_geno(rng, 30, 300, 2, miss=0.05)
+        nall[::7] = 1
+        geno[:, :, ::7] = np.where(geno[:, :, ::7] >= 0, 0, -9)
+        nall[::11] = 0
+        geno[:, :, ::11] = -9
+        A = 2
     K = 4
     och, gch = _pair(geno, nall, K, A, seed=3, chain=2, freqscorr=freqscorr)
     gch.init()
