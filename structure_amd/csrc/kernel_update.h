@@ -385,18 +385,20 @@ __global__ __launch_bounds__(kNT) void k_update_q_gibbs(const ChainDesc *__restr
     if (active) {
         const size_t n = (size_t)m.N * K;
         int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-        if (C.q_direct) {   // few locus tiles: their partials, four loads in flight
+        if (C.q_direct) {   // few locus tiles (<= kQDirectRows): all their partials in flight
             const SGPU_G uint16_t *f = g(C.nQpart) + (size_t)i * K + k;
             const int nrows = C.n_lt;
-            int t = 0;
-            for (; t + 4 <= nrows; t += 4) {
-                c0 += f[(size_t)t * n];
-                c1 += f[(size_t)(t + 1) * n];
-                c2 += f[(size_t)(t + 2) * n];
-                c3 += f[(size_t)(t + 3) * n];
+            int32_t c[kQDirectRows];
+#pragma unroll
+            for (int t = 0; t < kQDirectRows; t++) {
+                c[t] = t < nrows ? (int32_t)f[(size_t)t * n] : 0;
             }
-            for (; t < nrows; t++) {
-                c0 += f[(size_t)t * n];
+#pragma unroll
+            for (int t = 0; t < kQDirectRows; t += 4) {
+                c0 += c[t];
+                c1 += c[t + 1];
+                c2 += c[t + 2];
+                c3 += c[t + 3];
             }
         } else {            // the kRowSplit rows of the fold
             const SGPU_G int32_t *f = g(C.nQfold) + (size_t)i * K + k;
@@ -993,8 +995,8 @@ __device__ __forceinline__ double like_share(const ChainDesc &C, int s, int ns)
     return wave_sum(lane_sum8(g(C.likepart) + b0, b1 - b0, 1, 0, (int)(threadIdx.x & 63), 64));
 }
 
-// Flags of the epsilon launch.
-constexpr int kEpsCollect = 1, kEpsFst = 2, kEpsAlpha = 4, kEpsLike = 8;
+// Flags of the epsilon launch (kEpsAdvance: it closes the sweep, see k_update_eps).
+constexpr int kEpsCollect = 1, kEpsFst = 2, kEpsAlpha = 4, kEpsLike = 8, kEpsAdvance = 16;
 
 // The kRoleScalars block of a chain in the epsilon launch (kNT threads): the
 // lambda decision of a chain without the F model (nothing reads lambda between
@@ -1052,12 +1054,10 @@ __device__ void scalars_block(const ChainDesc &C, const ModelArgs &m, uint32_t r
 // sweep's ancestry-count partials for the next sweep's Gibbs update, and one
 // kRoleScalars block per chain runs alpha, the lnP(D) sum and the scalar sums.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict__ cds,
-                                                     const BlockRef *__restrict__ map,
-                                                     const uint32_t *__restrict__ repp, int flags)
+__device__ void eps_launch_block(const ChainDesc *__restrict__ cds, const BlockRef *__restrict__ map,
+                                 uint32_t rep, int flags)
 {
     SGPU_BLOCK_PROLOGUE();
-    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
     if (br_.role == kRoleFold) {
         fold_rows_block(C, m, bx, (int)br_.by);
         return;
@@ -1195,6 +1195,34 @@ __global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict_
         }
     } else if ((int)threadIdx.x < K) {
         part[(size_t)bx * K + threadIdx.x] = s_f[threadIdx.x];
+    }
+}
+
+// Closing a sweep (kEpsAdvance, when no Fst decision follows): the block that
+// finishes last (counted in *done, which it resets) advances the device sweep
+// index; every block has read the index by then.
+__device__ __forceinline__ void advance_when_last(uint32_t *repd, uint32_t *done, uint32_t rep)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t prev =
+            __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev + 1u == gridDim.x) {
+            __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(repd, rep + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kNT) void k_update_eps(const ChainDesc *__restrict__ cds,
+                                                     const BlockRef *__restrict__ map,
+                                                     uint32_t *__restrict__ repd,
+                                                     uint32_t *__restrict__ done, int flags)
+{
+    const uint32_t rep = *repd;   // sweep index, read on the device (graph replays)
+    eps_launch_block(cds, map, rep, flags);
+    if (flags & kEpsAdvance) {
+        advance_when_last(repd, done, rep);
     }
 }
 
@@ -1398,15 +1426,7 @@ __global__ __launch_bounds__(kTailNT) void k_sweep_tail(const ChainDesc *__restr
         }
     }
     if (flags & kTailAdvance) {
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint32_t prev =
-                __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (prev + 1u == gridDim.x) {
-                __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(repd, rep + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
+        advance_when_last(repd, done, rep);
     }
 }
 

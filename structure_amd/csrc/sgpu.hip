@@ -65,8 +65,6 @@ constexpr int kZBlocksDefault = 4096;
 constexpr int kMetroBlocksDefault = 1024;
 // LDS per UpdateZ block for the per-individual counts (keeps 3 blocks per CU).
 constexpr size_t kZLdsMax = 48 * 1024;
-// Most UpdateZ locus tiles whose ancestry partials the Gibbs update sums itself.
-constexpr int kQDirectRows = 32;
 // Most loci one Metropolis-ratio block walks.
 constexpr long long kMetroMaxLoci = 2048;
 // LDS per byte-format UpdateZ block for the staged frequencies and counts, and
@@ -147,7 +145,7 @@ struct sgpu_ctx {
     int64_t nsum = 0;
     bool initialized = false;
     bool like_ready = false;
-    bool tail_advances = false;       // the last sweep's tail launch advances *repd
+    bool tail_advances = false;       // the last sweep's closing launch advances *repd
     int byte_rows = kByteRows;        // block size of the byte-format UpdateZ launches
     int8_t *zdbg = nullptr;
     std::vector<void *> allocs;
@@ -1610,7 +1608,7 @@ int sgpu_update_eps(sgpu_ctx *c, uint32_t rep, int32_t collect)
         return rc;
     }
     PhaseTimer timer(c, SGPU_PH_EPS);
-    return launch(c, k_update_eps, c->p_eps, kNT, (const uint32_t *)c->repd,
+    return launch(c, k_update_eps, c->p_eps, kNT, c->repd, c->repd + 1,
                   collect ? kEpsCollect : 0);
 }
 
@@ -1670,17 +1668,22 @@ int sgpu_sweep(sgpu_ctx *c, uint32_t rep)
             return rc;
         }
     }
-    const int ef = (collect ? kEpsCollect : 0) | (c->any_freqscorr ? kEpsFst : 0) |
-                   (c->any_alpha ? kEpsAlpha : 0) | ((like && c->like_ready) ? kEpsLike : 0);
+    // in a captured sweep the last launch (Fst decision, else epsilon) advances
+    // the device sweep index
+    c->tail_advances = false;
+    int ef = (collect ? kEpsCollect : 0) | (c->any_freqscorr ? kEpsFst : 0) |
+             (c->any_alpha ? kEpsAlpha : 0) | ((like && c->like_ready) ? kEpsLike : 0);
     if (c->any_freqscorr || c->p_fold.nblocks > 0 || (ef & (kEpsAlpha | kEpsCollect))) {
+        if (c->capturing && c->p_tail.nblocks == 0) {
+            ef |= kEpsAdvance;
+        }
         PhaseTimer timer(c, SGPU_PH_EPS);
-        if ((rc = launch(c, k_update_eps, c->p_postz, kNT, (const uint32_t *)c->repd, ef))) {
+        if ((rc = launch(c, k_update_eps, c->p_postz, kNT, c->repd, c->repd + 1, ef))) {
             return rc;
         }
         c->fold_fresh = true;
+        c->tail_advances = (ef & kEpsAdvance) != 0;
     }
-    // in a captured sweep the Fst launch advances the device sweep index
-    c->tail_advances = false;
     if (c->p_tail.nblocks > 0) {
         const int tf = kTailFst | (collect ? kTailCollect : 0) | (c->capturing ? kTailAdvance : 0);
         PhaseTimer timer(c, SGPU_PH_COLLECT);

@@ -28,6 +28,8 @@ constexpr double kLn2 = 0.6931471805599453;
 constexpr int kZLoci = 256;        // loci per UpdateZ block (one locus per lane)
 constexpr int kMaxK = 20;          // largest K the kernels take (MAXPOPS; UpdateZ.c:41-46)
 constexpr int kMaxAlleles = 255;   // byte format
+// Most UpdateZ locus tiles whose ancestry partials the Gibbs update sums itself.
+constexpr int kQDirectRows = 32;
 
 struct ModelArgs {
     int N, L, K, A;
