@@ -226,7 +226,10 @@ int sgpu_update_fst(sgpu_ctx *ctx, uint32_t rep);
  * UpdateZ ran with SGPU_Z_LIKE, like += ..., sumlikes, sumsqlikes. */
 int sgpu_collect(sgpu_ctx *ctx, uint32_t rep);
 
-/* One sweep of the hot loop (structure.c:497-638) in reference phase order. */
+/* One sweep of the hot loop (structure.c:497-638), with the reference's
+ * draws and results.  The phases run in five launches (seven with a
+ * Metropolis Q update): alpha, lambda and the Fst terms ride in the launches
+ * of phases whose inputs and outputs they do not touch. */
 int sgpu_sweep(sgpu_ctx *ctx, uint32_t rep);
 
 /* n sweeps rep0 .. rep0+n-1. */
@@ -234,7 +237,10 @@ int sgpu_run_sweeps(sgpu_ctx *ctx, uint32_t rep0, uint32_t n);
 
 /* ---- phase timing ------------------------------------------------------- */
 /* The reference's queue is created without profiling (Kernels.c:691); these
- * record HIP events around every phase launch on the context stream. */
+ * record HIP events around every phase launch on the context stream.  In a
+ * sweep, lambda's log sums count under P (F model) or Q, its decision under Q
+ * (F model) or EPS, alpha and the Fst terms under EPS, the Fst decision under
+ * COLLECT; ALPHA, LAMBDA and FST time the stand-alone calls. */
 enum {
     SGPU_PH_P = 0, SGPU_PH_Q = 1, SGPU_PH_Z = 2, SGPU_PH_ALPHA = 3, SGPU_PH_LAMBDA = 4,
     SGPU_PH_EPS = 5, SGPU_PH_FST = 6, SGPU_PH_COLLECT = 7, SGPU_NPHASE = 8
