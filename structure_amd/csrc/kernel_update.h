@@ -540,9 +540,11 @@ template <int K>
 constexpr int metro_loci() { return K <= 16 ? 256 : 128; }
 
 // Biallelic 2-bit path, K templated: the block's P columns are staged in LDS
-// (wave-uniform reads), four loci per genotype load, fused multiply-adds in
-// the dot products (the ratio feeds an exp() compared with a uniform; it
-// needs no reference-exact rounding).
+// allele-major ([locus][allele][pop]), four loci per genotype load; each copy
+// reads its own allele's K frequencies (the lanes of a wave read at most two
+// addresses, in different banks) and forms its two dot products with fused
+// multiply-adds (the ratio feeds an exp() compared with a uniform; it needs no
+// reference-exact rounding).
 template <int K>
 __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__restrict__ cds,
                                                            const BlockRef *__restrict__ map,
@@ -565,7 +567,8 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__res
         cq[k] = active ? Q[(size_t)i * K + k] : 1.0;
     }
     constexpr int kMetroLoci = metro_loci<K>();
-    __shared__ double s_p[kMetroLoci * K * 2];
+    constexpr int KP = (K + 1) & ~1;   // an allele's row padded to 16 bytes (paired reads)
+    __shared__ __align__(16) double s_p[kMetroLoci * KP * 2];
     const int l0 = by * loci_per_range;
     const int l1 = min(l0 + loci_per_range, m.L);
     const SGPU_G uint32_t *row = geno + (size_t)(active ? (i >> 4) : 0) * Lpad;
@@ -576,7 +579,9 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__res
         const int c1 = min(c0 + kMetroLoci, l1);
         __syncthreads();
         for (int t = threadIdx.x; t < (c1 - c0) * K * 2; t += kNT) {
-            s_p[t] = P[(size_t)c0 * K * 2 + t];
+            // P[l][k][a] -> s_p[l - c0][a][k], rows of KP
+            const int lt = t / (2 * K), r = t - lt * 2 * K;
+            s_p[lt * 2 * KP + (r & 1) * KP + (r >> 1)] = P[(size_t)c0 * K * 2 + t];
         }
         __syncthreads();
         // four loci per load; the next four are loaded one iteration ahead
@@ -590,25 +595,25 @@ __global__ __launch_bounds__(kNT) void k_metro_ratio_snp2(const ChainDesc *__res
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int l = lb + u;
-                if (l < c1) {
-                    const double *Pl = s_p + (l - c0) * K * 2;
-                    double tp0 = 0.0, tp1 = 0.0, tm0 = 0.0, tm1 = 0.0;
+                const uint32_t code = (ws[u] >> sh) & 3u;
+                if (l < c1 && code != 3u) {
+                    // copy 0 carries allele 1 only in code 2, copy 1 in codes 1 and 2
+                    const double *Pl = s_p + (l - c0) * KP * 2;
+                    const double *p0 = Pl + (code == 2u ? KP : 0);
+                    const double *p1 = Pl + (code != 0u ? KP : 0);
+                    double t0 = 0.0, t1 = 0.0, m0 = 0.0, m1 = 0.0;
 #pragma unroll
                     for (int k = 0; k < K; k++) {
-                        const double p0 = Pl[2 * k], p1 = Pl[2 * k + 1];
-                        tp0 = fma(tq[k], p0, tp0);
-                        tp1 = fma(tq[k], p1, tp1);
-                        tm0 = fma(cq[k], p0, tm0);
-                        tm1 = fma(cq[k], p1, tm1);
+                        const double a = p0[k], b = p1[k];
+                        t0 = fma(tq[k], a, t0);
+                        t1 = fma(tq[k], b, t1);
+                        m0 = fma(cq[k], a, m0);
+                        m1 = fma(cq[k], b, m1);
                     }
-                    const uint32_t code = (ws[u] >> sh) & 3u;
-                    if (code != 3u) {
-                        const bool x0 = code == 2u, x1 = code != 0u;
-                        pp *= x0 ? tp1 : tp0;
-                        pp *= x1 ? tp1 : tp0;
-                        pm *= x0 ? tm1 : tm0;
-                        pm *= x1 ? tm1 : tm0;
-                    }
+                    pp *= t0;
+                    pp *= t1;
+                    pm *= m0;
+                    pm *= m1;
                 }
             }
             int e;
