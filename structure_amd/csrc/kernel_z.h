@@ -115,6 +115,18 @@ constexpr int z_snp2_waves()
     return K == 5 ? 6 : K == 7 ? 5 : (K == 10 || K == 11) ? 4 : (K >= 14 && K <= 16) ? 3 : 1;
 }
 
+// Individual pairs per iteration of the 2-bit UpdateZ's pair loop: two, so that
+// one pair's Philox rounds overlap the other's compares.  Measured at N=2000,
+// L=100000 (ms per launch, one pair -> two): K=2 0.337 -> 0.332, 4 0.466 -> 0.460,
+// 5 0.535 -> 0.529, 8 0.781 -> 0.777, 10 0.931 -> 0.926, 17 2.082 -> 2.067; K=20
+// 2.537 -> 2.550.  One pair at K = 11, 15 and 16, whose tighter register budgets
+// above would spill the second pair, and from K = 18 on.
+template <int K>
+constexpr int z_snp2_pair_unroll()
+{
+    return (K == 11 || K == 15 || K == 16 || K >= 18) ? 1 : 2;
+}
+
 template <int K, bool INIT, bool LIKE, bool ZOUT, bool PF>
 __global__ __launch_bounds__(kZLoci) __attribute__((amdgpu_waves_per_eu(z_snp2_waves<K>())))
 void k_update_z_snp2(const ChainDesc *__restrict__ cds,
@@ -144,6 +156,7 @@ void k_update_z_snp2(const ChainDesc *__restrict__ cds,
     // [tile individual pair][K]: valid copies, then z > k; the pair's even individual
     // in the low 16 bits, the odd one in the high 16 bits
     extern __shared__ uint32_t s_cnt[];
+    constexpr int kPairUnroll = z_snp2_pair_unroll<K>();
     const int lane = threadIdx.x & 63;
     const int l = bx * kZLoci + threadIdx.x;   // < Lpad by construction
     const bool lok = l < L;
@@ -201,6 +214,7 @@ void k_update_z_snp2(const ChainDesc *__restrict__ cds,
         for (int k = 0; k < K; k++) {
             col[k] = 0;
         }
+#pragma unroll kPairUnroll
         for (int jp = 0; jp < 8; jp++) {
             const int i0 = g * 16 + 2 * jp;
             if (i0 >= N) {
