@@ -61,6 +61,65 @@ __device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
     return u32x4{c0, c1, c2, c3};
 }
 
+// The same Philox4x32-10 block when c0, c2, c3 and the key are wave-uniform and
+// only c1 varies across lanes (UpdateZ: sweep, individual pair and phase per
+// wave, locus per lane).  In the first rounds some products and XOR operands are
+// uniform; grouping the uniform operands lets them combine on the scalar unit,
+// where a three-input XOR with two scalar operands would first copy one into a
+// vector register.  The words are those of philox4x32_10 bit for bit.
+__device__ __forceinline__ u32x4 philox4x32_10_lane1(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                     uint32_t c3, uint32_t k0, uint32_t k1)
+{
+    constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    // round 1: both products uniform; only c1 is per lane
+    uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+    uint32_t a0 = c1 ^ ((uint32_t)(p1 >> 32) ^ k0);            // lane
+    uint32_t a2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;             // uniform
+    uint32_t a1 = (uint32_t)p1, a3 = (uint32_t)p0;            // uniform
+    k0 += W0;
+    k1 += W1;
+    // round 2: p0 per lane, p1 uniform
+    p0 = (uint64_t)M0 * a0;
+    p1 = (uint64_t)M1 * a2;
+    uint32_t b0 = (uint32_t)(p1 >> 32) ^ a1 ^ k0;             // uniform
+    uint32_t b2 = (uint32_t)(p0 >> 32) ^ (a3 ^ k1);           // lane
+    uint32_t b1 = (uint32_t)p1, b3 = (uint32_t)p0;            // uniform, lane
+    k0 += W0;
+    k1 += W1;
+    // round 3: p0 uniform, p1 per lane
+    p0 = (uint64_t)M0 * b0;
+    p1 = (uint64_t)M1 * b2;
+    uint32_t d0 = (uint32_t)(p1 >> 32) ^ (b1 ^ k0);           // lane
+    uint32_t d2 = b3 ^ ((uint32_t)(p0 >> 32) ^ k1);           // lane
+    uint32_t d1 = (uint32_t)p1, d3 = (uint32_t)p0;            // lane, uniform
+    k0 += W0;
+    k1 += W1;
+    // round 4: both products per lane, c3 still uniform
+    p0 = (uint64_t)M0 * d0;
+    p1 = (uint64_t)M1 * d2;
+    c0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), d1, k0, 0x96);
+    c2 = (uint32_t)(p0 >> 32) ^ (d3 ^ k1);
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    k0 += W0;
+    k1 += W1;
+    // rounds 5..10: everything per lane
+#pragma unroll
+    for (int r = 4; r < 10; r++) {
+        const uint64_t q0 = (uint64_t)M0 * c0;
+        const uint64_t q1 = (uint64_t)M1 * c2;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(q1 >> 32), c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(q0 >> 32), c3, k1, 0x96);
+        c1 = (uint32_t)q1;
+        c3 = (uint32_t)q0;
+        c0 = n0;
+        c2 = n2;
+        k0 += W0;
+        k1 += W1;
+    }
+    return u32x4{c0, c1, c2, c3};
+}
+
 __device__ __forceinline__ uint32_t word_of(const u32x4 &w, int j)
 {
     return j == 0 ? w.x : (j == 1 ? w.y : (j == 2 ? w.z : w.w));

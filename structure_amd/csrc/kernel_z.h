@@ -48,10 +48,12 @@ __device__ __forceinline__ uint32_t write_lane(uint32_t v, uint32_t value, int l
 // running product, CalcLike.c:113-126, without the periodic logs).
 // max(t, c) for the finite non-negative terms here, without the NaN
 // canonicalisation the compiler adds to fmax.
+// The bound c is wave-uniform (a constant): it stays in scalar registers, so the
+// loop does not copy it into vector registers before each use.
 __device__ __forceinline__ double max_raw(double t, double c)
 {
     double r;
-    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(t), "v"(c));
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(t), "s"(c));
     return r;
 }
 
@@ -220,8 +222,8 @@ void k_update_z_snp2(const ChainDesc *__restrict__ cds,
             if (i0 >= N) {
                 break;
             }
-            const u32x4 w = philox4x32_10(rep, (uint32_t)l, (uint32_t)(i0 >> 1),
-                                          (INIT ? PH_INITZ : PH_Z) << 24, seed, chain);
+            const u32x4 w = philox4x32_10_lane1(rep, (uint32_t)l, (uint32_t)(i0 >> 1),
+                                                (INIT ? PH_INITZ : PH_Z) << 24, seed, chain);
             uint32_t nkp[K];   // the pair's counts, individual i0 low, i0 + 1 high
 #pragma unroll
             for (int k = 0; k < K; k++) {
@@ -234,7 +236,7 @@ void k_update_z_snp2(const ChainDesc *__restrict__ cds,
                     break;
                 }
                 const int t = 2 * jp + h;
-                const uint32_t code = (word >> (2 * t)) & 3u;
+                const uint32_t code = __builtin_amdgcn_ubfe(word, 2u * (uint32_t)t, 2u);
                 const bool valid = code != 3u;
                 const bool x0 = code == 2u;
                 const bool x1 = code != 0u;
