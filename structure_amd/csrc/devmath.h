@@ -120,6 +120,54 @@ __device__ __forceinline__ u32x4 philox4x32_10_lane1(uint32_t c0, uint32_t c1, u
     return u32x4{c0, c1, c2, c3};
 }
 
+// The same Philox4x32-10 block when only c2 varies across lanes (byte-format
+// UpdateZ: sweep, locus and phase per wave, individual pair per lane); as
+// philox4x32_10_lane1, the uniform operands of the first rounds are grouped.
+__device__ __forceinline__ u32x4 philox4x32_10_lane2(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                     uint32_t c3, uint32_t k0, uint32_t k1)
+{
+    constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    // round 1: p0 uniform, p1 per lane
+    uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+    uint32_t a0 = (uint32_t)(p1 >> 32) ^ (c1 ^ k0);           // lane
+    uint32_t a2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;             // uniform
+    uint32_t a1 = (uint32_t)p1, a3 = (uint32_t)p0;            // lane, uniform
+    k0 += W0;
+    k1 += W1;
+    // round 2: p0 per lane, p1 uniform
+    p0 = (uint64_t)M0 * a0;
+    p1 = (uint64_t)M1 * a2;
+    uint32_t b0 = a1 ^ ((uint32_t)(p1 >> 32) ^ k0);           // lane
+    uint32_t b2 = (uint32_t)(p0 >> 32) ^ (a3 ^ k1);           // lane
+    uint32_t b1 = (uint32_t)p1, b3 = (uint32_t)p0;            // uniform, lane
+    k0 += W0;
+    k1 += W1;
+    // round 3: both products per lane, c1 uniform
+    p0 = (uint64_t)M0 * b0;
+    p1 = (uint64_t)M1 * b2;
+    c0 = (uint32_t)(p1 >> 32) ^ (b1 ^ k0);
+    c2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), b3, k1, 0x96);
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    k0 += W0;
+    k1 += W1;
+    // rounds 4..10: everything per lane
+#pragma unroll
+    for (int r = 3; r < 10; r++) {
+        const uint64_t q0 = (uint64_t)M0 * c0;
+        const uint64_t q1 = (uint64_t)M1 * c2;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(q1 >> 32), c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(q0 >> 32), c3, k1, 0x96);
+        c1 = (uint32_t)q1;
+        c3 = (uint32_t)q0;
+        c0 = n0;
+        c2 = n2;
+        k0 += W0;
+        k1 += W1;
+    }
+    return u32x4{c0, c1, c2, c3};
+}
+
 __device__ __forceinline__ uint32_t word_of(const u32x4 &w, int j)
 {
     return j == 0 ? w.x : (j == 1 ? w.y : (j == 2 ? w.z : w.w));

@@ -522,7 +522,7 @@ void k_update_z_byte(const ChainDesc *__restrict__ cds,
             const uint32_t word = nextw;
             nextw = (rowok && j + 1 < nc) ? geno[(size_t)(l + 1) * Gpad + h] : 0xffffffffu;
             if (word != 0xffffffffu) {
-                const u32x4 w = philox4x32_10(rep, (uint32_t)l, (uint32_t)h,
+                const u32x4 w = philox4x32_10_lane2(rep, (uint32_t)l, (uint32_t)h,
                                               (INIT ? PH_INITZ : PH_Z) << 24, seed, chain);
                 const double *s_pl = s_p + (size_t)j * KA;
                 uint32_t *s_hl = s_h + (size_t)j * KA;
