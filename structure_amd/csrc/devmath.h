@@ -276,7 +276,7 @@ __device__ __forceinline__ double cos_d(double a)
 
 __device__ __forceinline__ double box_muller(uint32_t w0, uint32_t w1)
 {
-    double r = sqrt(-2.0 * log_d(uo(w0)));
+    double r = sqrt(-2.0 * log_pos(uo(w0)));   // uo() lies in (0, 1)
     return r * cos_d(6.283185307179586 * uo(w1));
 }
 
@@ -299,7 +299,8 @@ __device__ __forceinline__ bool mt_attempt(double x, double u, double d, double 
     }
     v = v * v * v;
     const double x2 = x * x;
-    if (u < 1.0 - 0.0331 * (x2 * x2) || log_d(u) < 0.5 * x2 + d * (1.0 - v + log_d(v))) {
+    // u in (0, 1) and v > 0 here: the logs need no special cases
+    if (u < 1.0 - 0.0331 * (x2 * x2) || log_pos(u) < 0.5 * x2 + d * (1.0 - v + log_pos(v))) {
         *g = d * v;
         return true;
     }
@@ -327,7 +328,7 @@ __device__ __forceinline__ double gamma_draw(double a, uint32_t seed, uint32_t c
     bool done = false;
     for (uint32_t j = 0; j < kGammaMaxBlocks && !done; j++) {
         const u32x4 w = philox4x32_10(c0, c1, c2, (phase << 24) | j, seed, chain);
-        const double rad = sqrt(-2.0 * log_d(uo(w.x)));
+        const double rad = sqrt(-2.0 * log_pos(uo(w.x)));
         double sn, cs;
         sincos_d(6.283185307179586 * uo(w.y), &sn, &cs);
         done = mt_attempt(rad * cs, uo(w.z), d, c, &g) || mt_attempt(rad * sn, uo(w.w), d, c, &g);
@@ -337,7 +338,7 @@ __device__ __forceinline__ double gamma_draw(double a, uint32_t seed, uint32_t c
     }
     if (a < 1.0) {
         const u32x4 b = philox4x32_10(c0, c1, c2, (phase << 24) | kGammaBoostBlock, seed, chain);
-        g = g * exp(log_d(uo(b.x)) / a);
+        g = g * exp(log_pos(uo(b.x)) / a);
     }
     return g;
 }
