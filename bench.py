@@ -45,7 +45,11 @@ def sweep_bytes(N, L, K, A):
 
 
 CPU_SHARE = 16   # host cores one GPU's runs may use on the benchmark host
-MIN_WARM = 30    # warm-up sweeps of the single-chain lines, at least
+# Warm-up sweeps, at least.  With METROFREQ=10 the library replays graphs of 20
+# sweeps (two Metropolis periods) and single-sweep graphs around them: 30 warm-up
+# sweeps from rep 0 capture a 20-sweep graph (reps 0-19) and both single-sweep
+# graphs (20-29), so the timed sweeps, from rep 30 on, capture nothing.
+MIN_WARM = 30
 
 
 def cpu_info():
@@ -210,9 +214,10 @@ def lnp_error_per_k(words, N, L, finals):
 def _timed_run(ch, warmup, steps, direct=False):
     """Warm-up sweeps, then `steps` timed sweeps from graph replay (or, with
     `direct`, launched one kernel at a time); returns seconds.  The chains
-    collect from sweep 0 (BURNIN=0): the warm-up, at least two sweeps (rep 0 is
-    a Metropolis sweep, rep 1 a Gibbs one), captures both sweep graphs the timed
-    sweeps replay, so no capture falls inside the timed region."""
+    collect from sweep 0 (BURNIN=0); a warm-up of MIN_WARM sweeps captures every
+    graph the timed sweeps replay, so no capture falls inside the timed region
+    (shorter warm-ups, as the config-3 line's, start the timed region off a
+    Metropolis sweep and replay only single-sweep graphs, both captured by then)."""
     if direct:
         ch.set_timing(True)   # timing mode launches every phase directly
     ch.init()
@@ -435,7 +440,7 @@ def workload_c5(args):
     specs, data = _ragged_datasets(args.seed, keep_geno=not args.no_check or not args.no_cpu)
     print(f"bench: 64 ragged data sets in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
           flush=True)
-    warm, sweeps = max(args.warmup, 2), args.steps
+    warm, sweeps = max(args.warmup, MIN_WARM), args.steps
     models = [sa.make_model(sp["N"], sp["L"], sp["K"], dd[3], freqscorr=1, inferalpha=1,
                             metrofreq=10, burnin=0, numreps=sweeps, seed=args.seed + j, chain=j)
               for j, (sp, dd) in enumerate(zip(specs, data))]
@@ -446,7 +451,8 @@ def workload_c5(args):
     sites = sum(sp["N"] * sp["L"] * sp["K"] for sp in specs)
     res = {"metric": "chain-sweeps/s, 64 ragged data sets in one batch",
            "value": round(batched, 3), "unit": "chain-sweeps/s", "n_gpus": 1, "steps": sweeps,
-           "warmup": warm, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "warmup": warm, "warmup_requested": args.warmup, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": f"synthetic ragged (seed {args.seed})",
            "config": {"workload": "config 5: 64 data sets, N~U[100,5000], L log-uniform "
                                   "[1e3,2e5], half 2-bit SNP / half byte microsatellite "
@@ -478,7 +484,7 @@ def workload_c1batch(args):
                           seed=args.seed)
     fmt, words = sa.pack(d["geno"])
     A = d["A"]
-    warm, sweeps = max(args.warmup, 2), args.steps
+    warm, sweeps = max(args.warmup, MIN_WARM), args.steps
     models = [sa.make_model(200, 500, 3, A, freqscorr=0, inferalpha=1, lambda0=1.0,
                             metrofreq=10, burnin=0, numreps=sweeps, seed=args.seed + r,
                             chain=3) for r in range(100)]
@@ -487,7 +493,8 @@ def workload_c1batch(args):
                                                 direct=args.direct)
     res = {"metric": "chain-sweeps/s, 100 replicate chains of config 1 in one batch",
            "value": round(batched, 3), "unit": "chain-sweeps/s", "n_gpus": 1, "steps": sweeps,
-           "warmup": warm, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "warmup": warm, "warmup_requested": args.warmup, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": f"synthetic microsatellites (seed {args.seed})",
            "config": {"workload": "config 1 x 100 seeds: N=200 L=500 A in [2,10] K=3, "
                                   "no F model, alpha inferred", "chains": 100},
@@ -548,10 +555,10 @@ def main():
     d = sa.synth_snp(N, L, Ktrue=5, flo=0.01, fhi=0.1, qalpha=0.3, miss=0.01, seed=args.seed)
     print(f"bench: synthetic data N={N} L={L} in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
           flush=True)
-    # every sweep collects (BURNIN=0): the warm-up, at least a Metropolis sweep (rep 0)
-    # and a Gibbs one (rep 1), captures both graphs that the timed sweeps replay.  It
-    # runs at least MIN_WARM sweeps: right after start-up the first 20 sweeps run 6%
-    # slower than the next (clocks and caches warming), so fewer would time the ramp.
+    # every sweep collects (BURNIN=0): the warm-up, at least MIN_WARM sweeps,
+    # captures every graph that the timed sweeps replay; right after start-up the
+    # first 20 sweeps also run 6% slower than the next (clocks and caches warming),
+    # so fewer would time the ramp.
     warm = max(args.warmup, MIN_WARM)
     model = sa.make_model(N, L, K, 2, freqscorr=1, inferalpha=1, inferlambda=1, metrofreq=10,
                           burnin=0, numreps=args.steps, seed=args.seed, chain=0)

@@ -227,19 +227,25 @@ int sgpu_update_fst(sgpu_ctx *ctx, uint32_t rep);
 int sgpu_collect(sgpu_ctx *ctx, uint32_t rep);
 
 /* One sweep of the hot loop (structure.c:497-638), with the reference's
- * draws and results.  The phases run in five launches (seven with a
- * Metropolis Q update): alpha, lambda and the Fst terms ride in the launches
- * of phases whose inputs and outputs they do not touch. */
+ * draws and results.  The phases run in four launches (six with a Metropolis
+ * Q update): the Gibbs Q update, alpha, lambda and the Fst terms ride in the
+ * launches of phases whose inputs and outputs they do not touch. */
 int sgpu_sweep(sgpu_ctx *ctx, uint32_t rep);
 
-/* n sweeps rep0 .. rep0+n-1. */
+/* n sweeps rep0 .. rep0+n-1, the same draws and results as n calls of
+ * sgpu_sweep, replayed from HIP graphs captured on first use: graphs of
+ * several sweeps (whole Metropolis periods, or 16 Gibbs sweeps) where a span
+ * lies wholly in burn-in or in the collecting sweeps, single-sweep graphs
+ * around them. */
 int sgpu_run_sweeps(sgpu_ctx *ctx, uint32_t rep0, uint32_t n);
 
 /* ---- phase timing ------------------------------------------------------- */
 /* The reference's queue is created without profiling (Kernels.c:691); these
  * record HIP events around every phase launch on the context stream.  In a
- * sweep, lambda's log sums count under P (F model) or Q, its decision under Q
- * (F model) or EPS, alpha and the Fst terms under EPS, the Fst decision under
+ * sweep, the Gibbs Q update counts under P (Metropolis Q updates under Q);
+ * lambda's decision under P (F model) or EPS, the log sums of P it reads under
+ * Q in Metropolis sweeps and LAMBDA in Gibbs sweeps (no F model), the log-eps
+ * sums (F model), alpha and the Fst terms under EPS, the Fst decision under
  * COLLECT; ALPHA, LAMBDA and FST time the stand-alone calls. */
 enum {
     SGPU_PH_P = 0, SGPU_PH_Q = 1, SGPU_PH_Z = 2, SGPU_PH_ALPHA = 3, SGPU_PH_LAMBDA = 4,

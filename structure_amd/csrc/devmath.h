@@ -343,22 +343,27 @@ __device__ __forceinline__ double gamma_draw(double a, uint32_t seed, uint32_t c
     return g;
 }
 
-// Term-by-term Lanczos sum in the host routine's order (mymath.c:382-429).
+// Term-by-term Lanczos sum in the host routine's order (mymath.c:382-429):
+// c_8 / (z + 7) first, then down to c_1 / z.  The loop stays rolled, one
+// quotient at a time: the branch is all but never taken, and unrolled its eight
+// independent divisions would set the register budget of every kernel that
+// calls lgamma_lanczos.
 __device__ __forceinline__ double lgamma_lanczos_terms(double z)
 {
-    const double c1 = 676.5203681218835, c2 = -1259.139216722289, c3 = 771.3234287757674,
-                 c4 = -176.6150291498386, c5 = 12.50734324009056, c6 = -0.1385710331296526,
-                 c7 = 9.934937113930748e-6, c8 = 1.659470187408462e-7;
-    // denominators z+7 .. z+0 by repeated subtraction, as the host routine does
-    double s = 0.0, den = z + 7.0;
-    s += c8 / den; den -= 1.0;
-    s += c7 / den; den -= 1.0;
-    s += c6 / den; den -= 1.0;
-    s += c5 / den; den -= 1.0;
-    s += c4 / den; den -= 1.0;
-    s += c3 / den; den -= 1.0;
-    s += c2 / den; den -= 1.0;
-    s += c1 / den;
+    double s = 0.0, den = z + 7.0;   // denominators by repeated subtraction, as the host
+#pragma unroll 1
+    for (int j = 7; j >= 0; j--) {
+        const double c = j == 7 ? 1.659470187408462e-7
+                         : j == 6 ? 9.934937113930748e-6
+                         : j == 5 ? -0.1385710331296526
+                         : j == 4 ? 12.50734324009056
+                         : j == 3 ? -176.6150291498386
+                         : j == 2 ? 771.3234287757674
+                         : j == 1 ? -1259.139216722289
+                                  : 676.5203681218835;
+        s += c / den;
+        den -= 1.0;
+    }
     s += 0.9999999999995183;
     return log_pos(s) + 0.9189385332046727 - (z + 6.5) + (z - 0.5) * log_pos(z + 6.5);
 }

@@ -18,7 +18,10 @@ namespace sgpu {
 constexpr int kNT = 256;   // threads per block for the update kernels
 
 __device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint32_t rep,
-                                   int nparts);
+                                   const double *parts, int nparts);
+template <int NF>
+__device__ void gibbs_q_block(const ChainDesc &C, const ModelArgs &m, int bx, uint32_t rep,
+                              int collect);
 
 // The chain and block coordinates of this block (uniform: scalar loads).
 #define SGPU_BLOCK_PROLOGUE()                          \
@@ -145,19 +148,27 @@ __global__ __launch_bounds__(kNT) void k_lambda_partial(const ChainDesc *__restr
 // group's sum in allele order from LDS and LogRDirichletDisc's rule
 // (ran.c:1247-1272) including the sum < UNDERFLO uniform fallback.
 // Fused: PSum += P (output.c:673-677) when collecting.  In a sweep the launch
-// also carries kRoleLamPart blocks: the lambda log sums of F-model chains read
-// only eps, which nothing writes between here and the lambda decision.
+// also carries the lambda decision of F-model chains (kRoleLambda: it reads the
+// log-eps sums of the last epsilon pass, and neither UpdateP under the F model
+// nor UpdateQ reads lambda) and, in Gibbs sweeps, the Gibbs Q update
+// (kRoleQGibbs: it reads the counts of the last UpdateZ and alpha, not P).
+// Five waves per SIMD, as UpdateP alone needs (93 VGPRs): the lambda decision
+// would raise the kernel to 113, and spills only in its own one-block path.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNT) void k_update_p(const ChainDesc *__restrict__ cds,
-                                                   const BlockRef *__restrict__ map,
-                                                   const uint32_t *__restrict__ repp, int collect)
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(5))) void k_update_p(
+    const ChainDesc *__restrict__ cds, const BlockRef *__restrict__ map,
+    const uint32_t *__restrict__ repp, int collect)
 {
     SGPU_BLOCK_PROLOGUE();
-    if (br_.role == kRoleLamPart) {
-        lambda_partial_block(C, m, bx);
+    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    if (br_.role == kRoleLambda) {
+        lambda_final_block(C, m, rep, C.lam_epart, C.e_blocks);
         return;
     }
-    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
+    if (br_.role == kRoleQGibbs) {
+        gibbs_q_block<8>(C, m, bx, rep, collect);   // few loads in flight: UpdateP's registers
+        return;
+    }
     const SGPU_G int32_t *nall = g(C.nall);
     const SGPU_G uint16_t *nApart = g(C.nApart);
     const SGPU_G double *eps = g(C.eps);
@@ -356,21 +367,11 @@ __global__ __launch_bounds__(kNT) void k_fold_rows(const ChainDesc *__restrict__
 // over the block's individuals of log Q[i][k] (LogProbQonepop,
 // UpdateAlpha.c:102-111), and the sum of the per-individual totals of
 // log max(Q, 1e-50) (LogProbQ, UpdateAlpha.c:157-180).
-__global__ __launch_bounds__(kNT) void k_update_q_gibbs(const ChainDesc *__restrict__ cds,
-                                                         const BlockRef *__restrict__ map,
-                                                         const uint32_t *__restrict__ repp,
-                                                         int collect)
+template <int NF>
+__device__ void gibbs_q_block(const ChainDesc &C, const ModelArgs &m, int bx, uint32_t rep,
+                              int collect)
 {
-    SGPU_BLOCK_PROLOGUE();
-    const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
-    if (br_.role == kRoleLambda) {   // the chain's lambda decision (F model)
-        lambda_final_block(C, m, rep, C.l_blocks);
-        return;
-    }
-    if (br_.role == kRoleLamPart) {  // lambda log sums of the new P (no F model)
-        lambda_partial_block(C, m, bx);
-        return;
-    }
+    static_assert(NF % 4 == 0 && kQDirectRows % NF == 0, "partials in flight");
     const SGPU_G double *alpha = g(C.alpha);
     SGPU_G double *Q = g(C.Q);
     SGPU_G double *QSum = g(C.QSum);
@@ -385,20 +386,23 @@ __global__ __launch_bounds__(kNT) void k_update_q_gibbs(const ChainDesc *__restr
     if (active) {
         const size_t n = (size_t)m.N * K;
         int32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-        if (C.q_direct) {   // few locus tiles (<= kQDirectRows): all their partials in flight
+        if (C.q_direct) {   // few locus tiles (<= kQDirectRows): NF partials in flight
             const SGPU_G uint16_t *f = g(C.nQpart) + (size_t)i * K + k;
             const int nrows = C.n_lt;
-            int32_t c[kQDirectRows];
+#pragma unroll 1
+            for (int t0 = 0; t0 < nrows; t0 += NF) {
+                int32_t c[NF];
 #pragma unroll
-            for (int t = 0; t < kQDirectRows; t++) {
-                c[t] = t < nrows ? (int32_t)f[(size_t)t * n] : 0;
-            }
+                for (int t = 0; t < NF; t++) {
+                    c[t] = t0 + t < nrows ? (int32_t)f[(size_t)(t0 + t) * n] : 0;
+                }
 #pragma unroll
-            for (int t = 0; t < kQDirectRows; t += 4) {
-                c0 += c[t];
-                c1 += c[t + 1];
-                c2 += c[t + 2];
-                c3 += c[t + 3];
+                for (int t = 0; t < NF; t += 4) {
+                    c0 += c[t];
+                    c1 += c[t + 1];
+                    c2 += c[t + 2];
+                    c3 += c[t + 3];
+                }
             }
         } else {            // the kRowSplit rows of the fold
             const SGPU_G int32_t *f = g(C.nQfold) + (size_t)i * K + k;
@@ -461,6 +465,15 @@ __global__ __launch_bounds__(kNT) void k_update_q_gibbs(const ChainDesc *__restr
     }
 }
 
+__global__ __launch_bounds__(kNT) void k_update_q_gibbs(const ChainDesc *__restrict__ cds,
+                                                         const BlockRef *__restrict__ map,
+                                                         const uint32_t *__restrict__ repp,
+                                                         int collect)
+{
+    SGPU_BLOCK_PROLOGUE();
+    gibbs_q_block<kQDirectRows>(C, m, bx, *repp, collect);
+}
+
 // Log-Q statistics of the current Q (after initialisation or a host write of Q).
 __global__ __launch_bounds__(kNT) void k_logq_stats(const ChainDesc *__restrict__ cds,
                                                     const BlockRef *__restrict__ map)
@@ -505,10 +518,6 @@ __global__ __launch_bounds__(kNT) void k_metro_propose(const ChainDesc *__restri
 {
     SGPU_BLOCK_PROLOGUE();
     const uint32_t rep = *repp;   // sweep index, read on the device (graph replays)
-    if (br_.role == kRoleLambda) {   // the chain's lambda decision (F model)
-        lambda_final_block(C, m, rep, C.l_blocks);
-        return;
-    }
     if (br_.role == kRoleLamPart) {  // lambda log sums of the new P (no F model)
         lambda_partial_block(C, m, bx);
         return;
@@ -872,9 +881,9 @@ __global__ __launch_bounds__(kNT) void k_update_alpha(const ChainDesc *__restric
 // the normaliser terms of the allele-count groups in parallel (thread h takes
 // groups h, h + kNT, ...) and thread 0 adds them in thread order and decides.
 __device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint32_t rep,
-                                   int nparts)
+                                   const double *parts, int nparts)
 {
-    const SGPU_G double *part = g(C.lam_part);
+    const SGPU_G double *part = g(parts);
     const SGPU_G int32_t *ahist = g(C.ahist);
     SGPU_G double *lambda = g(C.lambda);
     const int nhist = C.nhist;
@@ -898,16 +907,31 @@ __device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint3
         if (!((nw > 0.0) && (nw < kLambdaMax))) {   // block-uniform
             continue;
         }
+        // one Lanczos sum at a time (the rolled loops), which keeps this block
+        // within the register budget of the launches that carry it
         double loc = 0.0;
-        for (int h = (int)threadIdx.x; h < nhist; h += kNT) {
-            const double na = (double)ahist[2 * h];
-            const double cnt = (double)ahist[2 * h + 1];
-            double t = 0.0;
-            t += stoppop * lgamma_lanczos(na * nw);
-            t -= stoppop * lgamma_lanczos(na * old);
-            t += stoppop * na * lgamma_lanczos(old);
-            t -= stoppop * na * lgamma_lanczos(nw);
-            loc += cnt * t;
+        if ((int)threadIdx.x < nhist) {
+            double lg_old = 0.0, lg_nw = 0.0;   // the terms that do not depend on the group
+#pragma unroll 1
+            for (int w = 0; w < 2; w++) {
+                const double v = lgamma_lanczos(w == 0 ? old : nw);
+                lg_old = w == 0 ? v : lg_old;
+                lg_nw = w == 0 ? lg_nw : v;
+            }
+#pragma unroll 1
+            for (int h = (int)threadIdx.x; h < nhist; h += kNT) {
+                const double na = (double)ahist[2 * h];
+                const double cnt = (double)ahist[2 * h + 1];
+                double t = 0.0;
+#pragma unroll 1
+                for (int w = 0; w < 2; w++) {
+                    const double v = stoppop * lgamma_lanczos(na * (w == 0 ? nw : old));
+                    t = w == 0 ? t + v : t - v;
+                }
+                t += stoppop * na * lg_old;
+                t -= stoppop * na * lg_nw;
+                loc += cnt * t;
+            }
         }
         s_t[threadIdx.x] = loc;
         __syncthreads();
@@ -936,7 +960,7 @@ __global__ __launch_bounds__(kNT) void k_lambda_finalize(const ChainDesc *__rest
                                                           const uint32_t *__restrict__ repp)
 {
     SGPU_BLOCK_PROLOGUE();
-    lambda_final_block(C, m, *repp, C.l_blocks);
+    lambda_final_block(C, m, *repp, C.lam_part, C.l_blocks);
 }
 
 // Fst proposal of group pop1 (UpdateFst.c:160-166): F + FPRIORSD * N(0,1).
@@ -1014,8 +1038,8 @@ __device__ void scalars_block(const ChainDesc &C, const ModelArgs &m, uint32_t r
 {
     __shared__ double s_like[3];
     const int wid = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-    if (C.lam_mode == kLamP) {   // lambda, from the Q-update launch's log sums
-        lambda_final_block(C, m, rep, C.l_blocks);
+    if (C.lam_mode == kLamP) {   // lambda, from the log sums of the new P
+        lambda_final_block(C, m, rep, C.lam_part, C.l_blocks);
     }
     if (wid == 0 && (flags & kEpsAlpha) && m.inferalpha) {
         alpha_wave(C, m, rep);
@@ -1059,6 +1083,49 @@ __device__ void scalars_block(const ChainDesc &C, const ModelArgs &m, uint32_t r
 // sweep's ancestry-count partials for the next sweep's Gibbs update, and one
 // kRoleScalars block per chain runs alpha, the lnP(D) sum and the scalar sums.
 // ---------------------------------------------------------------------------
+// Pairwise trees over a block's loci (entries j * K + k for j < lpb), in place:
+// a[k] and b[k] end as the sums over j.
+__device__ __forceinline__ void loci_tree(double *a, double *b, int j, int K, int lpb)
+{
+    for (int st = 1; st < lpb; st *= 2) {
+        if (j < lpb && j % (2 * st) == 0 && j + st < lpb) {
+            a[threadIdx.x] += a[threadIdx.x + st * K];
+            b[threadIdx.x] += b[threadIdx.x + st * K];
+        }
+        __syncthreads();
+    }
+}
+
+// The log-eps sums of the epsilon pass's blocks (F-model lambda) for the eps
+// the chain holds now (after initialisation or a host write of eps): the same
+// loci per block, terms and tree as the epsilon pass, so the same values.
+__global__ __launch_bounds__(kNT) void k_eps_logsum(const ChainDesc *__restrict__ cds,
+                                                     const BlockRef *__restrict__ map)
+{
+    SGPU_BLOCK_PROLOGUE();
+    __shared__ double s_a[kNT], s_b[kNT];
+    const int K = m.K;
+    const int lpb = kNT / K;
+    const int j = threadIdx.x / K, k = threadIdx.x % K;
+    const int l = bx * lpb + j;
+    const bool live = j < lpb && l < m.L;
+    const int na = live ? g(C.nall)[l] : 0;
+    double lg = 0.0;
+    if (live && k == 0 && na > 1) {
+        const SGPU_G double *e = g(C.eps) + (size_t)l * m.A;
+        for (int aa = 0; aa < na; aa++) {
+            lg += log_d(e[aa]);
+        }
+    }
+    s_a[threadIdx.x] = 0.0;
+    s_b[threadIdx.x] = lg;
+    __syncthreads();
+    loci_tree(s_a, s_b, j, K, lpb);
+    if (threadIdx.x == 0) {
+        g(C.lam_epart)[bx] = s_b[0];
+    }
+}
+
 __device__ void eps_launch_block(const ChainDesc *__restrict__ cds, const BlockRef *__restrict__ map,
                                  uint32_t rep, int flags)
 {
@@ -1079,6 +1146,7 @@ __device__ void eps_launch_block(const ChainDesc *__restrict__ cds, const BlockR
     SGPU_G double *epsSum = g(C.epsSum);
     __shared__ double s_t[kNT];         // each (locus, pop)'s part of the move's log-ratio
     __shared__ double s_f[kNT];         // Fst terms of each (locus, pop)
+    __shared__ double s_g[kNT];         // log-eps sums of the loci (F-model lambda)
     __shared__ double s_newf[kMaxK];    // this sweep's Fst proposals
     const int K = m.K;
     const int lpb = kNT / K;                       // loci per block
@@ -1150,12 +1218,21 @@ __device__ void eps_launch_block(const ChainDesc *__restrict__ cds, const BlockR
             }
         }
     }
-    if (!fst) {
+    // F-model lambda: the log-eps sum of the block's loci as this pass leaves
+    // eps, for the next sweep's lambda decision
+    const bool lamsum = C.lam_mode == kLamEps;   // block-uniform
+    if (!fst && !lamsum) {
         return;
+    }
+    double lg = 0.0;
+    if (lamsum && live && k == 0 && na > 1) {
+        for (int aa = 0; aa < na; aa++) {
+            lg += log_d(e[aa]);
+        }
     }
     double v = 0.0;
     const int grp = m.onefst ? 0 : k;
-    const double newf = s_newf[grp];
+    const double newf = fst ? s_newf[grp] : 0.0;
     if (live && na > 0 && newf > 0.0 && newf < 1.0) {
         const double oldf = F[grp];
         const double nfr = (1.0 - newf) / newf;
@@ -1182,12 +1259,14 @@ __device__ void eps_launch_block(const ChainDesc *__restrict__ cds, const BlockR
     }
     // per group, a pairwise tree over the block's loci
     s_f[threadIdx.x] = v;
+    s_g[threadIdx.x] = lg;
     __syncthreads();
-    for (int st = 1; st < lpb; st *= 2) {
-        if (j < lpb && j % (2 * st) == 0 && j + st < lpb) {
-            s_f[threadIdx.x] += s_f[threadIdx.x + st * K];
-        }
-        __syncthreads();
+    loci_tree(s_f, s_g, j, K, lpb);
+    if (lamsum && threadIdx.x == 0) {
+        g(C.lam_epart)[bx] = s_g[0];
+    }
+    if (!fst) {
+        return;
     }
     SGPU_G double *part = g(C.fst_part);
     if (m.onefst) {

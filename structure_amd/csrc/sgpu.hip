@@ -62,6 +62,10 @@ size_t cdiv(size_t a, size_t b) { return (a + b - 1) / b; }
 // the 2-bit UpdateZ is measured fastest near 4096 at config 2, where 2048
 // leaves the second round of blocks on the CUs a third full).
 constexpr int kZBlocksDefault = 4096;
+// Sweeps per span graph: the whole Metropolis periods that make at least this
+// many, or this many Gibbs sweeps when there is no Metropolis sweep or the
+// Metropolis period is longer.
+constexpr int kPeriodSweeps = 16;
 constexpr int kMetroBlocksDefault = 1024;
 // LDS per UpdateZ block for the per-individual counts (keeps 3 blocks per CU).
 constexpr size_t kZLdsMax = 48 * 1024;
@@ -125,15 +129,22 @@ struct sgpu_ctx {
     std::vector<KGroup> z_groups, metro_groups;
     Plan p_update_p, p_fold, p_gibbs, p_logq, p_accept, p_alpha, p_lam_part,
         p_lam_fin, p_eps, p_fst_part, p_fst_fin, p_collect, p_init_eps;
-    // the fused launches of a sweep: UpdateP with the lambda log-sum blocks of
-    // F-model chains; the Q update with their lambda decision blocks and the
-    // log-sum blocks of the other chains; the epsilon launch with the scalar
+    // the fused launches of a sweep: UpdateP with the lambda decision blocks of
+    // F-model chains, in Gibbs sweeps also with the Gibbs Q blocks
+    // (p_update_pq_sw); in Metropolis sweeps the proposal launch with the lambda
+    // log-sum blocks of chains without the F model, which in Gibbs sweeps run
+    // in a launch of their own (p_lam_part_p); the epsilon launch with the scalar
     // blocks, epsilon + Fst blocks and count-fold blocks; the Fst decision
     // (F-model chains); separate lambda launches for pop-specific lambda with F
-    Plan p_update_p_sw, p_gibbs_sw, p_postz, p_tail, p_lam_part_x, p_lam_fin_x;
+    Plan p_update_p_sw, p_update_pq_sw, p_propose_sw, p_lam_part_p, p_postz, p_tail,
+        p_lam_part_x, p_lam_fin_x;
+    Plan p_eps_logsum;   // the log-eps sums of F-model lambda chains, outside a sweep
     // nQfold holds the fold of the current nQpart (false after UpdateZ or a host
     // write of the ancestry counts, until a fold runs)
     bool fold_fresh = false;
+    // lam_epart holds the log-eps sums of the current eps (false after
+    // initialisation or a host write of eps, until k_eps_logsum runs)
+    bool lam_fresh = false;
     std::vector<Plan> p_logq_chain;   // per chain, after a host write of Q
     // device sweep index read by every kernel
     uint32_t *repd = nullptr;
@@ -142,6 +153,8 @@ struct sgpu_ctx {
     bool capturing = false;
     bool use_graphs = true;
     hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};   // [metro | 2*collect]
+    hipGraphExec_t period_graph[2] = {nullptr, nullptr};              // [collect]
+    uint32_t period_sweeps = kPeriodSweeps;   // sweeps per period graph
     int64_t nsum = 0;
     bool initialized = false;
     bool like_ready = false;
@@ -951,6 +964,9 @@ int setup_chain(sgpu_ctx *c, HostChain &h, const Dataset &ds, const BatchTiles &
     if ((rc = build_p_slots(c, h, ds))) {
         return rc;
     }
+    if (d.lam_mode == kLamEps && (rc = dalloc(c, &d.lam_epart, (size_t)d.e_blocks))) {
+        return rc;
+    }
     return dalloc(c, &d.lam_part, (size_t)d.l_blocks * K);
 }
 
@@ -1009,21 +1025,30 @@ int build_plans(sgpu_ctx *c)
             return std::make_pair((int)cdiv(h.m.N, kNT / h.m.K), 1);
         };
         auto mode = [](int lm) { return [lm](const HostChain &h) { return h.d.lam_mode == lm; }; };
+        auto p_dims = [](const HostChain &h) { return std::make_pair(h.d.p_blocks, 1); };
         std::vector<BlockRef> v;
-        add_blocks(c, v, all_chains,
-                   [](const HostChain &h) { return std::make_pair(h.d.p_blocks, 1); });
-        add_blocks(c, v, mode(kLamEps), lb, kRoleLamPart);
+        add_blocks(c, v, mode(kLamEps), one, kRoleLambda);
+        add_blocks(c, v, all_chains, p_dims);
         if ((rc = upload_plan(c, c->p_update_p_sw, v, kXFast))) {
             return rc;
         }
         v.clear();
         add_blocks(c, v, mode(kLamEps), one, kRoleLambda);
-        add_blocks(c, v, all_chains, gibbs_dims);
-        add_blocks(c, v, mode(kLamP), lb, kRoleLamPart);
-        if ((rc = upload_plan(c, c->p_gibbs_sw, v, kXFast))) {
+        add_blocks(c, v, all_chains, gibbs_dims, kRoleQGibbs);
+        add_blocks(c, v, all_chains, p_dims);
+        if ((rc = upload_plan(c, c->p_update_pq_sw, v, kXFast))) {
             return rc;
         }
-        if ((rc = build_plan(c, c->p_lam_part_x, mode(kLamSplit), lb)) ||
+        v.clear();
+        add_blocks(c, v, all_chains, gibbs_dims);
+        add_blocks(c, v, mode(kLamP), lb, kRoleLamPart);
+        if ((rc = upload_plan(c, c->p_propose_sw, v, kXFast))) {
+            return rc;
+        }
+        if ((rc = build_plan(c, c->p_lam_part_p, mode(kLamP), lb)) ||
+            (rc = build_plan(c, c->p_eps_logsum, mode(kLamEps),
+                             [](const HostChain &h) { return std::make_pair(h.d.e_blocks, 1); })) ||
+            (rc = build_plan(c, c->p_lam_part_x, mode(kLamSplit), lb)) ||
             (rc = build_plan(c, c->p_lam_fin_x, mode(kLamSplit), one))) {
             return rc;
         }
@@ -1202,6 +1227,9 @@ int sgpu_create_batch(const sgpu_model *models, int32_t nchains, const sgpu_data
     }
     c->burnin = models[0].burnin;
     c->metrofreq = models[0].metrofreq;
+    c->period_sweeps = (uint32_t)(c->metrofreq > 0 && c->metrofreq <= kPeriodSweeps
+                                      ? c->metrofreq * ((kPeriodSweeps + c->metrofreq - 1) / c->metrofreq)
+                                      : kPeriodSweeps);
     c->computeprob = models[0].computeprob;
 
     int rc;
@@ -1281,6 +1309,12 @@ int sgpu_destroy(sgpu_ctx *c)
             c->graph[t] = nullptr;
         }
     }
+    for (int t = 0; t < 2; t++) {
+        if (c->period_graph[t]) {
+            (void)hipGraphExecDestroy(c->period_graph[t]);
+            c->period_graph[t] = nullptr;
+        }
+    }
     if (c->stream) {
         (void)hipStreamSynchronize(c->stream);
     }
@@ -1341,6 +1375,7 @@ int sgpu_init_state(sgpu_ctx *c)
     if (rc) {
         return rc;
     }
+    c->lam_fresh = false;
     c->rep_known = false;
     if ((rc = use_rep(c, 0)) || (rc = launch_z(c, 0)) || (rc = launch(c, k_logq_stats, c->p_logq, kNT))) {
         return rc;
@@ -1446,6 +1481,9 @@ int sgpu_set_chain_state(sgpu_ctx *c, int32_t chain, int32_t which, const void *
     }
     HIP_OK(hipMemcpyAsync(state_ptr(h.d, which), host, bytes, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
+    if (which == SGPU_EPS) {
+        c->lam_fresh = false;
+    }
     if (which == SGPU_Q) {
         return launch(c, k_logq_stats, c->p_logq_chain[chain], kNT);
     }
@@ -1466,7 +1504,8 @@ int sgpu_set_state(sgpu_ctx *c, int32_t which, const void *host, size_t bytes)
 
 namespace {
 
-// UpdateP over the plan pp: p_update_p, or p_update_p_sw with lambda log-sum blocks.
+// UpdateP over the plan pp: p_update_p, or a sweep's p_update_p_sw / p_update_pq_sw
+// with lambda decision and Gibbs Q blocks.
 int update_p(sgpu_ctx *c, uint32_t rep, int32_t collect, const Plan &pp)
 {
     int rc = check_ready(c);
@@ -1492,8 +1531,25 @@ int ensure_fold(sgpu_ctx *c)
     return SGPU_OK;
 }
 
-// UpdateQ over the (individual, pop) plan qp: p_gibbs, or p_gibbs_sw with the
-// lambda blocks of a sweep.
+// The log-eps sums that the lambda decision of F-model chains reads, unless
+// lam_epart already holds those of the current eps.
+int ensure_lam(sgpu_ctx *c)
+{
+    if (c->lam_fresh) {
+        return SGPU_OK;
+    }
+    if (c->p_eps_logsum.nblocks > 0) {
+        int rc = launch(c, k_eps_logsum, c->p_eps_logsum, kNT);
+        if (rc) {
+            return rc;
+        }
+    }
+    c->lam_fresh = true;
+    return SGPU_OK;
+}
+
+// UpdateQ over the (individual, pop) plan qp: p_gibbs, or in a Metropolis sweep
+// p_propose_sw with the lambda log-sum blocks.
 int update_q(sgpu_ctx *c, uint32_t rep, int32_t collect, const Plan &qp)
 {
     int rc = check_ready(c);
@@ -1648,18 +1704,33 @@ int sgpu_sweep(sgpu_ctx *c, uint32_t rep)
     if (rc) {
         return rc;
     }
-    // The reference's phase order (structure.c:497-638) in five launches.  The
-    // lambda update moves, as nothing between reads or writes what it reads and
-    // writes: F-model chains sum log eps beside UpdateP and decide beside UpdateQ
-    // (lambda is read next by the epsilon update); other chains sum log P beside
-    // UpdateQ and decide in the epsilon launch (lambda is read next by UpdateP).
-    // Then UpdateZ; epsilon with the Fst partials, the count fold for the next
-    // sweep, and per chain alpha (it reads only Q and alpha), the lnP(D) sum and
-    // the scalar sums; the Fst decision with FstSum, which may close the sweep.
+    // The reference's phase order (structure.c:497-638) in four launches (Gibbs
+    // sweeps) or six (Metropolis).  UpdateP and the Gibbs Q update share a launch:
+    // the Gibbs update reads the counts of the last UpdateZ and alpha, neither of
+    // which UpdateP reads or writes.  The lambda update moves, as nothing between
+    // reads or writes what it reads and writes: F-model chains decide in the
+    // UpdateP launch from the log-eps sums of the last epsilon pass (lambda is read
+    // next by the epsilon update, eps written first by it); other chains sum log P
+    // after UpdateP and decide in the epsilon launch (lambda is read next by
+    // UpdateP).  Then UpdateZ; epsilon with the Fst partials and the log-eps sums,
+    // the count fold for the next sweep, and per chain alpha (it reads only Q and
+    // alpha), the lnP(D) sum and the scalar sums; the Fst decision with FstSum,
+    // which may close the sweep.
     const int collect = ((int64_t)rep + 1 > c->burnin) ? 1 : 0;
     const bool like = collect && c->computeprob;
-    if ((rc = update_p(c, rep, collect, c->p_update_p_sw))) return rc;
-    if ((rc = update_q(c, rep, collect, c->p_gibbs_sw))) return rc;
+    const bool metro = c->metrofreq > 0 && rep % (uint32_t)c->metrofreq == 0;
+    if ((rc = ensure_lam(c))) return rc;
+    if (metro) {
+        if ((rc = update_p(c, rep, collect, c->p_update_p_sw))) return rc;
+        if ((rc = update_q(c, rep, collect, c->p_propose_sw))) return rc;
+    } else {
+        if ((rc = ensure_fold(c))) return rc;
+        if ((rc = update_p(c, rep, collect, c->p_update_pq_sw))) return rc;
+        if (c->p_lam_part_p.nblocks > 0) {   // lambda log sums of the new P (no F model)
+            PhaseTimer timer(c, SGPU_PH_LAMBDA);
+            if ((rc = launch(c, k_lambda_partial, c->p_lam_part_p, kNT))) return rc;
+        }
+    }
     if ((rc = sgpu_update_z(c, rep, like ? SGPU_Z_LIKE : 0, nullptr))) return rc;
     if (c->p_lam_fin_x.nblocks > 0) {   // pop-specific lambda with the F model
         PhaseTimer timer(c, SGPU_PH_LAMBDA);
@@ -1698,6 +1769,51 @@ int sgpu_sweep(sgpu_ctx *c, uint32_t rep)
     return SGPU_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Captures sweeps rep .. rep + span - 1 into one executable graph, each closed
+// by the advance of the device sweep index.  The host-side sweep count is left
+// as it was: the replays count.
+int capture_sweeps(sgpu_ctx *c, hipGraphExec_t *exec, uint32_t rep, uint32_t span)
+{
+    hipGraph_t g = nullptr;
+    const int64_t nsum0 = c->nsum;
+    HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    int rc = SGPU_OK;
+    for (uint32_t s = 0; s < span && rc == SGPU_OK; s++) {
+        rc = sgpu_sweep(c, rep + s);
+        if (rc == SGPU_OK && !c->tail_advances) {
+            hipLaunchKernelGGL(k_advance_rep, dim3(1), dim3(1), 0, c->stream, c->repd);
+        }
+    }
+    c->capturing = false;
+    c->nsum = nsum0;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) {
+            (void)hipGraphDestroy(g);
+        }
+        return rc;
+    }
+    if (e != hipSuccess) {
+        return fail(SGPU_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *exec = nullptr;
+        return fail(SGPU_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
+    }
+    return SGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int sgpu_run_sweeps(sgpu_ctx *c, uint32_t rep0, uint32_t n)
 {
     int rc = check_ready(c);
@@ -1712,54 +1828,48 @@ int sgpu_run_sweeps(sgpu_ctx *c, uint32_t rep0, uint32_t n)
         }
         return SGPU_OK;
     }
-    // One graph per sweep type (Metropolis or Gibbs Q, burn-in or collecting),
-    // captured on first use; each replay ends by advancing the device index.
-    // Every sweep leaves the count fold current for the next, so the graphs are
-    // captured (and replayed) with it current: fold now if it is not.
-    if ((rc = use_rep(c, rep0)) || (rc = ensure_fold(c))) {
+    // Graphs captured on first use and replayed: one per span of period_sweeps
+    // sweeps that lies wholly in burn-in or in the collecting sweeps and starts
+    // at a Metropolis sweep (periods up to kPeriodSweeps) or holds none (longer
+    // periods, or none) -- one replay for the lot, as a graph launch costs
+    // about 10 us of idle GPU between replays -- and one per single sweep type
+    // (Metropolis or Gibbs Q, burn-in or collecting) for the sweeps around
+    // them.  Each captured sweep ends by advancing the device index.  Every
+    // sweep leaves the count fold and the log-eps sums current for the next, so
+    // the graphs are captured (and replayed) with them current: make them now
+    // if they are not.
+    if ((rc = use_rep(c, rep0)) || (rc = ensure_fold(c)) || (rc = ensure_lam(c))) {
         return rc;
     }
-    for (uint32_t r = 0; r < n; r++) {
+    const uint32_t S = c->period_sweeps;
+    for (uint32_t r = 0; r < n;) {
         const uint32_t rep = rep0 + r;
-        const bool metro = c->metrofreq > 0 && rep % (uint32_t)c->metrofreq == 0;
         const bool collect = (int64_t)rep + 1 > c->burnin;
-        const int type = (metro ? 1 : 0) | (collect ? 2 : 0);
-        if (!c->graph[type]) {
-            hipGraph_t g = nullptr;
-            HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            c->capturing = true;
-            rc = sgpu_sweep(c, rep);
-            if (rc == SGPU_OK && !c->tail_advances) {
-                hipLaunchKernelGGL(k_advance_rep, dim3(1), dim3(1), 0, c->stream, c->repd);
-            }
-            c->capturing = false;
-            hipError_t e = hipStreamEndCapture(c->stream, &g);
-            if (rc) {
-                if (g) {
-                    (void)hipGraphDestroy(g);
-                }
-                return rc;
-            }
-            if (e != hipSuccess) {
-                return fail(SGPU_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
-            }
-            e = hipGraphInstantiate(&c->graph[type], g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) {
-                c->graph[type] = nullptr;
-                return fail(SGPU_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
-            }
-            // the captured sweep counted itself; the replay below counts again
-            if (collect) {
-                c->nsum--;
-            }
+        const uint32_t mf = c->metrofreq > 0 ? (uint32_t)c->metrofreq : 0u;
+        const bool aligned = mf == 0 || (mf <= (uint32_t)kPeriodSweeps
+                                             ? rep % mf == 0
+                                             : rep % mf != 0 && rep % mf + S <= mf);
+        const bool one_side = collect || (int64_t)rep + S <= c->burnin;
+        hipGraphExec_t *exec;
+        uint32_t span;
+        if (n - r >= S && aligned && one_side) {
+            exec = &c->period_graph[collect ? 1 : 0];
+            span = S;
+        } else {
+            const bool metro = c->metrofreq > 0 && rep % (uint32_t)c->metrofreq == 0;
+            exec = &c->graph[(metro ? 1 : 0) | (collect ? 2 : 0)];
+            span = 1;
         }
-        HIP_OK(hipGraphLaunch(c->graph[type], c->stream));
-        c->rep_shadow = rep + 1;
+        if (!*exec && (rc = capture_sweeps(c, exec, rep, span))) {
+            return rc;
+        }
+        HIP_OK(hipGraphLaunch(*exec, c->stream));
+        c->rep_shadow = rep + span;
         if (collect) {
-            c->nsum++;
+            c->nsum += span;
         }
         c->like_ready = collect && c->computeprob;
+        r += span;
     }
     return SGPU_OK;
 }

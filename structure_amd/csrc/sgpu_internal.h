@@ -80,6 +80,7 @@ struct ChainDesc {
     // lambda log-sum partials [l_blocks][K]; Fst log-ratio partials: [l_blocks][K]
     // (Fst pass) or [e_blocks][K] (epsilon pass)
     double *lam_part, *fst_part;
+    double *lam_epart;             // [e_blocks] log-eps sums of the last epsilon pass (kLamEps)
     double *QSum, *PSum, *epsSum, *FstSum, *alphaSum, *lambdaSum, *like3;
     int8_t *zout;                  // debug copy of Z [N][2][L] or null
     int32_t *adist;                // [N][K][numboxes] ANCESTDIST histogram, or null
@@ -117,10 +118,11 @@ struct BlockRef {
 // Block roles of the launches that carry more than one pass.
 enum : uint32_t {
     kRoleMain = 0,      // the launch's own pass
-    kRoleLambda = 1,    // Q-update launch: the lambda decision of the chain
+    kRoleLambda = 1,    // UpdateP launch: the lambda decision of an F-model chain
     kRoleFold = 2,      // epsilon launch: one block of the ancestry-count fold
     kRoleScalars = 3,   // epsilon launch: alpha, the lnP(D) sum and the scalar posterior sums
-    kRoleLamPart = 4    // UpdateP or Q-update launch: 256 loci of the lambda log sums
+    kRoleLamPart = 4,   // Metropolis proposal launch: 256 loci of the lambda log sums of P
+    kRoleQGibbs = 5     // UpdateP launch of a Gibbs sweep: a block of the Gibbs Q update
 };
 
 // Where a sweep runs a chain's lambda update (UpdateLambda.c:8-102): its log
@@ -128,8 +130,13 @@ enum : uint32_t {
 // epsilon update, which reads lambda.
 enum : int {
     kLamNone = 0,    // lambda not inferred
-    kLamEps = 1,     // F model: log sums in the UpdateP launch, decision in the Q-update launch
-    kLamP = 2,       // no F model: log sums in the Q-update launch, decision in the epsilon launch
+    // F model: the log-eps sums of the eps that each epsilon pass leaves, the
+    // decision in the next sweep's UpdateP launch (nothing before the epsilon
+    // update reads lambda or writes eps)
+    kLamEps = 1,
+    // no F model: log sums of the new P after UpdateP (the Metropolis proposal
+    // launch, or a launch of their own in Gibbs sweeps), decision in the epsilon launch
+    kLamP = 2,
     kLamSplit = 3    // F model with pop-specific lambda (log sums of the new P): own launches
 };
 

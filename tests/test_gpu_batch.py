@@ -35,9 +35,9 @@ CHAINS = [  # (data set, K, options)
 ]
 
 
-def _models(dsets, common):
+def _models(dsets, common, chains=CHAINS):
     out = []
-    for di, K, kw in CHAINS:
+    for di, K, kw in chains:
         _, _, _, N, L, A = dsets[di]
         o = dict(common)
         o.update(kw)
@@ -45,24 +45,21 @@ def _models(dsets, common):
     return out
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_batch_equals_chains_run_alone(graphs):
-    dsets = _datasets()
+def _batch_vs_alone(dsets, chains, graphs, nsweeps=17):
     common = dict(burnin=6, metrofreq=4)
-    models = _models(dsets, common)
-    batch = sa.Batch(models, [d[:3] for d in dsets], [c[0] for c in CHAINS])
-    assert len(batch) == len(CHAINS)
+    models = _models(dsets, common, chains)
+    batch = sa.Batch(models, [d[:3] for d in dsets], [c[0] for c in chains])
+    assert len(batch) == len(chains)
     batch.init()
-    nsweeps = 17
     if graphs:
         batch.run(0, 9)
         batch.run(9, nsweeps - 9)
     else:
         for r in range(nsweeps):
             batch.sweep(r)
-    for ci, (di, K, kw) in enumerate(CHAINS):
+    for ci, (di, K, kw) in enumerate(chains):
         fmt, words, nall = dsets[di][:3]
-        alone = sa.Chain(_models(dsets, common)[ci], fmt, words, nall)
+        alone = sa.Chain(_models(dsets, common, chains)[ci], fmt, words, nall)
         alone.init()
         alone.run(0, nsweeps)
         for sel in EXACT:
@@ -72,6 +69,34 @@ def test_batch_equals_chains_run_alone(graphs):
         assert la[1] != 0.0
         alone.close()
     batch.close()
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_batch_equals_chains_run_alone(graphs):
+    _batch_vs_alone(_datasets(), CHAINS, graphs)
+
+
+# Chains whose sweeps place phases in different launches: lambda per pop under
+# the F model (its own launches after UpdateZ), lambda without the F model
+# (decided in the epsilon launch), more than 32 UpdateZ locus tiles (the Gibbs
+# update reads the count fold of the previous epsilon launch), no F model and
+# no lambda (epsilon blocks absent), one Fst with pop alphas.
+MIXED = [
+    (2, 3, dict(seed=2, chain=0, inferlambda=1, popspecificlambda=1)),
+    (0, 4, dict(seed=2, chain=1, inferlambda=1, popspecificlambda=1)),
+    (0, 2, dict(seed=4, chain=2, freqscorr=0)),
+    (2, 2, dict(seed=5, chain=3, freqscorr=0, inferlambda=1)),
+    (1, 3, dict(seed=6, chain=4, onefst=1, inferalpha=1, popalphas=1)),
+    (2, 4, dict(seed=9, chain=5, inferalpha=1)),
+]
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_batch_of_mixed_launch_roles_equals_chains_run_alone(graphs):
+    big = sa.synth_snp(40, 9000, Ktrue=3, qalpha=0.5, miss=0.02, seed=23)
+    dsets = _datasets() + [(sa.GENO_SNP2, big["words"], big["nall"], 40, 9000, 2)]
+    print(f"mixed batch graphs={graphs}: {len(MIXED)} chains", flush=True)
+    _batch_vs_alone(dsets, MIXED, graphs, nsweeps=14)
 
 
 def test_batch_tuning_changes_tiles_not_results():

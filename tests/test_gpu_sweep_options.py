@@ -90,3 +90,46 @@ def test_sweep_options_follow_the_oracle(case, graphs):
     # lambda moved at least once, so the decision ran on real sums
     assert not np.all(gch.get(sa.LAMBDA) == 1.0)
     gch.close()
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_host_write_of_eps_between_sweeps(graphs):
+    """F-model lambda decides from the log-eps sums that the last epsilon pass
+    left; after a host write of eps the next sweep must sum the written eps.
+    Runs Gibbs and Metropolis sweeps on both sides of the write, with a count
+    fold (more than 32 locus tiles) on the chain, against the oracle."""
+    N, L, K = 40, 9000, 3
+    geno, nall, A, f = _data("snp", N, L, seed=5)
+    kw = dict(burnin=3, metrofreq=4, inferalpha=1, seed=91, chain=1, freqscorr=1, inferlambda=1)
+    gch = sa.Chain.from_geno(sa.make_model(N, L, K, A, **kw), geno, nall, f)
+    och = oracle.Chain(oracle.make_model(N, L, K, A, **kw), geno, nall)
+    print(f"host eps write, graphs={graphs}", flush=True)
+    gch.init()
+    och.init()
+    n1, n2 = 6, 7
+
+    def run(r0, n):
+        if graphs:
+            gch.run(r0, n)
+        else:
+            for rep in range(r0, r0 + n):
+                gch.sweep(rep)
+        for rep in range(r0, r0 + n):
+            och.sweep(rep)
+
+    run(0, n1)
+    rng = np.random.default_rng(8)
+    eps = np.zeros((L, A))
+    for l in range(L):
+        if nall[l] > 0:
+            eps[l, :nall[l]] = rng.dirichlet(np.ones(nall[l]))
+    gch.set(sa.EPS, eps)
+    och.eps[:] = eps
+    run(n1, n2)
+    assert np.array_equal(gch.get(sa.NA), och.nA)
+    assert np.array_equal(gch.get(sa.NQ), och.nQ)
+    for sel, ref in [(sa.Q, och.Q), (sa.P, och.P), (sa.EPS, och.eps), (sa.FST, och.F),
+                     (sa.LAMBDA, och.lambda_), (sa.LAMBDASUM, och.lambdaSum)]:
+        _close(gch.get(sel), ref)
+    assert not np.all(gch.get(sa.LAMBDA) == 1.0)
+    gch.close()
