@@ -129,6 +129,17 @@ constexpr int z_snp2_pair_unroll()
     return (K == 11 || K == 15 || K == 16 || K >= 18) ? 1 : 2;
 }
 
+// Burn-in sweeps (no lnP terms): each copy's r against both alleles' prefixes as
+// four products instead of four 64-bit selects and two products.  The two more
+// live doubles keep the register class (waves per SIMD, no new spill) at these
+// K; at K = 4, 7, 10, 11 and 14-16 they would cost a wave or spill, and those K
+// keep the selects.
+template <int K>
+constexpr bool z_snp2_burnin_products()
+{
+    return !(K == 4 || K == 7 || (K >= 10 && K <= 11) || (K >= 14 && K <= 16));
+}
+
 template <int K, bool INIT, bool LIKE, bool ZOUT, bool PF>
 __global__ __launch_bounds__(kZLoci) __attribute__((amdgpu_waves_per_eu(z_snp2_waves<K>())))
 void k_update_z_snp2(const ChainDesc *__restrict__ cds,
@@ -282,19 +293,30 @@ void k_update_z_snp2(const ChainDesc *__restrict__ cds,
                     // (all >= 0), so they never count.
                     const double sc = __hiloint2double(valid ? 0x3DF00000 : 0, 0);   // 2^-32 or 0
                     const double h0 = s0 * sc, h1 = s1 * sc;
-                    const double g0 = x0 ? h1 : h0, g1 = x1 ? h1 : h0;   // each copy's S * sc
-                    const double r0 = (double)wa * g0;
-                    const double r1 = (double)wb * g1;
-                    if (LIKE) {
-                        acc.mul_pair_scaled(g0, g1);
+                    // r of copy c against allele a's prefixes: ra* for allele 0, rb* for 1
+                    double ra0, rb0, ra1, rb1;
+                    if (LIKE || !z_snp2_burnin_products<K>()) {
+                        const double g0 = x0 ? h1 : h0, g1 = x1 ? h1 : h0;   // each copy's S * sc
+                        if (LIKE) {
+                            acc.mul_pair_scaled(g0, g1);
+                        }
+                        ra0 = rb0 = (double)wa * g0;
+                        ra1 = rb1 = (double)wb * g1;
+                    } else {
+                        // in the lanes whose mask below takes a compare, its r is the
+                        // same product as the selected one
+                        ra0 = (double)wa * h0;
+                        rb0 = (double)wa * h1;
+                        ra1 = (double)wb * h0;
+                        rb1 = (double)wb * h1;
                     }
                     // both allele sequences are compared and the copy's allele picks
                     // the lane mask in scalar registers
                     const uint64_t xm0 = ballot(x0), xm1 = ballot(x1);
 #pragma unroll
                     for (int k = 0; k < K - 1; k++) {
-                        const uint64_t a0 = ballot(pre0[k] < r0), b0 = ballot(pre1[k] < r0);
-                        const uint64_t a1 = ballot(pre0[k] < r1), b1 = ballot(pre1[k] < r1);
+                        const uint64_t a0 = ballot(pre0[k] < ra0), b0 = ballot(pre1[k] < rb0);
+                        const uint64_t a1 = ballot(pre0[k] < ra1), b1 = ballot(pre1[k] < rb1);
                         const uint64_t m0 = (xm0 & b0) | (~xm0 & a0);   // z0 > k
                         const uint64_t m1 = (xm1 & b1) | (~xm1 & a1);   // z1 > k
                         TA[k] += lane_select(m0 & fm, X0) + lane_select(m1 & fm, X1);
