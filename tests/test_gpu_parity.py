@@ -82,6 +82,24 @@ def test_update_z_counts_and_like_bitexact(kind, K):
         _close(like, och.calc_like(Q, P))
 
 
+@pytest.mark.parametrize("K", list(range(1, 21)))
+def test_burnin_update_z_counts_bitexact(K):
+    """UpdateZ without the lnP terms (burn-in sweeps), 2-bit words, every K up to
+    20: the variant forms each copy's scaled uniform against both alleles (four
+    products) at most K and keeps the selects at K = 4, 7, 10, 11 and 14-16; the
+    counts equal the oracle's either way."""
+    rng = np.random.default_rng(2000 + K)
+    geno, nall, A = _snp(70, 520, min(K, 5), seed=40 + K)
+    och, gch = _pair(geno, nall, K, A, fmt=sa.GENO_SNP2, seed=78, chain=4)
+    gch.init()
+    Q, P = _state(gch, nall, rng, K, A, fq=0.6)
+    for rep in (5, 6):
+        gch.update_z(rep, like=False)
+        nA, nQ = och.update_z(Q, P, rep)
+        assert np.array_equal(gch.get(sa.NA), nA), (K, rep)
+        assert np.array_equal(gch.get(sa.NQ), nQ), (K, rep)
+
+
 @pytest.mark.parametrize("kind", ["snp", "byte"])
 def test_init_state_matches_oracle(kind):
     if kind == "snp":
