@@ -108,13 +108,15 @@ struct LikeAcc {
 // a few registers fewer than its own allocation lift the occupancy a step and
 // the spills stay out of the inner loop; 1 = the compiler's choice.  Measured
 // per K at N=2000, L=100000 (ms per launch, compiler's choice -> this):
-// K=5 0.534 -> 0.525, 7 0.715 -> 0.678, 10 1.027 -> 0.939, 11 1.115 -> 1.084,
-// 14 1.712 -> 1.385, 15 1.910 -> 1.567, 16 2.027 -> 1.907; the same step made
-// K = 4, 6, 8, 9, 12, 13, 17..20 slower (spills inside the loop).
+// K=5 0.534 -> 0.525 (six waves), 7 0.715 -> 0.678, 10 1.027 -> 0.939, 11 1.115 ->
+// 1.084, 14 1.712 -> 1.385, 15 1.910 -> 1.567, 16 2.027 -> 1.907; the same step made
+// K = 4, 6, 8, 9, 12, 13, 17..20 slower (spills inside the loop).  K=5 takes a
+// second step to seven waves (72 VGPRs; the 8 spilled stay in the per-row code,
+// outside the pair loop): 0.5095 -> 0.5081 ms collecting, 0.474 -> 0.468 burn-in.
 template <int K>
 constexpr int z_snp2_waves()
 {
-    return K == 5 ? 6 : K == 7 ? 5 : (K == 10 || K == 11) ? 4 : (K >= 14 && K <= 16) ? 3 : 1;
+    return K == 5 ? 7 : K == 7 ? 5 : (K == 10 || K == 11) ? 4 : (K >= 14 && K <= 16) ? 3 : 1;
 }
 
 // Individual pairs per iteration of the 2-bit UpdateZ's pair loop: two, so that
