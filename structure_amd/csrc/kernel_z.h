@@ -112,11 +112,14 @@ struct LikeAcc {
 // 1.084, 14 1.712 -> 1.385, 15 1.910 -> 1.567, 16 2.027 -> 1.907; the same step made
 // K = 4, 6, 8, 9, 12, 13, 17..20 slower (spills inside the loop).  K=5 takes a
 // second step to seven waves (72 VGPRs; the 8 spilled stay in the per-row code,
-// outside the pair loop): 0.5095 -> 0.5081 ms collecting, 0.474 -> 0.468 burn-in.
+// outside the pair loop): 0.5095 -> 0.5081 ms collecting, 0.474 -> 0.468 burn-in;
+// K=6 likewise six waves (80 VGPRs, none spilled in the pair loop): 0.595 -> 0.593
+// collecting, 0.559 -> 0.553 burn-in.  At K = 8, 12, 13 and 17 the next step spills
+// inside the pair loop.
 template <int K>
 constexpr int z_snp2_waves()
 {
-    return K == 5 ? 7 : K == 7 ? 5 : (K == 10 || K == 11) ? 4 : (K >= 14 && K <= 16) ? 3 : 1;
+    return K == 5 ? 7 : K == 6 ? 6 : K == 7 ? 5 : (K == 10 || K == 11) ? 4 : (K >= 14 && K <= 16) ? 3 : 1;
 }
 
 // Individual pairs per iteration of the 2-bit UpdateZ's pair loop: two, so that
