@@ -234,9 +234,9 @@ int sgpu_sweep(sgpu_ctx *ctx, uint32_t rep);
 
 /* n sweeps rep0 .. rep0+n-1, the same draws and results as n calls of
  * sgpu_sweep, replayed from HIP graphs captured on first use: graphs of
- * several sweeps (whole Metropolis periods, or 16 Gibbs sweeps) where a span
- * lies wholly in burn-in or in the collecting sweeps, single-sweep graphs
- * around them. */
+ * several sweeps (whole Metropolis periods of up to 64 sweeps, else 16 Gibbs
+ * sweeps) where a span lies wholly in burn-in or in the collecting sweeps,
+ * single-sweep graphs around them. */
 int sgpu_run_sweeps(sgpu_ctx *ctx, uint32_t rep0, uint32_t n);
 
 /* ---- phase timing ------------------------------------------------------- */

@@ -62,10 +62,12 @@ size_t cdiv(size_t a, size_t b) { return (a + b - 1) / b; }
 // the 2-bit UpdateZ is measured fastest near 4096 at config 2, where 2048
 // leaves the second round of blocks on the CUs a third full).
 constexpr int kZBlocksDefault = 4096;
-// Sweeps per span graph: the whole Metropolis periods that make at least this
-// many, or this many Gibbs sweeps when there is no Metropolis sweep or the
-// Metropolis period is longer.
+// Sweeps per span graph: the whole Metropolis periods that make at least
+// kPeriodSweeps (for periods up to kMaxPeriodSweeps: one graph holds a period of
+// the reference's default METROFREQ = 20), or kPeriodSweeps Gibbs sweeps when
+// there is no Metropolis sweep or the period is longer.
 constexpr int kPeriodSweeps = 16;
+constexpr int kMaxPeriodSweeps = 64;
 constexpr int kMetroBlocksDefault = 1024;
 // LDS per UpdateZ block for the per-individual counts (keeps 3 blocks per CU).
 constexpr size_t kZLdsMax = 48 * 1024;
@@ -1227,7 +1229,7 @@ int sgpu_create_batch(const sgpu_model *models, int32_t nchains, const sgpu_data
     }
     c->burnin = models[0].burnin;
     c->metrofreq = models[0].metrofreq;
-    c->period_sweeps = (uint32_t)(c->metrofreq > 0 && c->metrofreq <= kPeriodSweeps
+    c->period_sweeps = (uint32_t)(c->metrofreq > 0 && c->metrofreq <= kMaxPeriodSweeps
                                       ? c->metrofreq * ((kPeriodSweeps + c->metrofreq - 1) / c->metrofreq)
                                       : kPeriodSweeps);
     c->computeprob = models[0].computeprob;
@@ -1830,7 +1832,7 @@ int sgpu_run_sweeps(sgpu_ctx *c, uint32_t rep0, uint32_t n)
     }
     // Graphs captured on first use and replayed: one per span of period_sweeps
     // sweeps that lies wholly in burn-in or in the collecting sweeps and starts
-    // at a Metropolis sweep (periods up to kPeriodSweeps) or holds none (longer
+    // at a Metropolis sweep (periods up to kMaxPeriodSweeps) or holds none (longer
     // periods, or none) -- one replay for the lot, as a graph launch costs
     // about 10 us of idle GPU between replays -- and one per single sweep type
     // (Metropolis or Gibbs Q, burn-in or collecting) for the sweeps around
@@ -1846,7 +1848,7 @@ int sgpu_run_sweeps(sgpu_ctx *c, uint32_t rep0, uint32_t n)
         const uint32_t rep = rep0 + r;
         const bool collect = (int64_t)rep + 1 > c->burnin;
         const uint32_t mf = c->metrofreq > 0 ? (uint32_t)c->metrofreq : 0u;
-        const bool aligned = mf == 0 || (mf <= (uint32_t)kPeriodSweeps
+        const bool aligned = mf == 0 || (mf <= (uint32_t)kMaxPeriodSweeps
                                              ? rep % mf == 0
                                              : rep % mf != 0 && rep % mf + S <= mf);
         const bool one_side = collect || (int64_t)rep + S <= c->burnin;

@@ -1,7 +1,7 @@
 """Sweeps replayed from graphs that span several sweeps (sgpu_run_sweeps):
-whole Metropolis periods from a Metropolis sweep, 16-sweep spans without one
-when the period is longer or there is none, and single-sweep graphs around
-them, across the burn-in boundary and over several run calls.  The replays
+whole Metropolis periods from a Metropolis sweep (periods up to 64 sweeps),
+16-sweep spans without one when the period is longer or there is none, and
+single-sweep graphs around them, across the burn-in boundary and over several run calls.  The replays
 launch the same kernels as direct sweeps (sgpu_sweep), so every state array,
 sum and lnP(D) is bit-identical."""
 import numpy as np
@@ -16,12 +16,13 @@ STATE = (sa.Q, sa.P, sa.EPS, sa.FST, sa.ALPHA, sa.LAMBDA, sa.NA, sa.NQ, sa.QSUM,
          sa.EPSSUM, sa.FSTSUM, sa.ALPHASUM, sa.LAMBDASUM, sa.NSUM, sa.LIKE)
 
 # (metrofreq, burnin, run calls as (rep0, n)): period graphs of 18 sweeps from
-# Metropolis sweeps (6, 24, ...); a 20-sweep period, so 16-sweep Gibbs spans
-# between Metropolis sweeps; no Metropolis sweeps, burn-in spans and collecting
-# spans
+# Metropolis sweeps (6, 24, ...); one 20-sweep period per graph; a 70-sweep
+# period, so 16-sweep Gibbs spans between Metropolis sweeps; no Metropolis
+# sweeps, burn-in spans and collecting spans
 CASES = {
     "period3": (3, 5, [(0, 7), (7, 40), (47, 13)]),
     "period20": (20, 10, [(0, 30), (30, 45)]),
+    "period70": (70, 10, [(0, 30), (30, 60), (90, 5)]),
     "gibbs_only": (0, 17, [(0, 50)]),
 }
 
