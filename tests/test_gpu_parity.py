@@ -50,7 +50,7 @@ def _state(gch, nall, rng, K, A, fq=1.0):
 
 CASES_Z = [("snp", 1), ("snp", 2), ("snp", 5), ("snp", 10), ("byte", 3), ("byte", 7),
            ("byteraw", 4), ("byteraw", 12),
-           # MAXPOPS above 16, up to the 20 of the north star (UpdateZ.c:41-46)
+           # MAXPOPS above 16, up to 20 (UpdateZ.c:41-46)
            ("snp", 17), ("snp", 20), ("byte", 17), ("byte", 20), ("byteraw", 19)]
 
 
