@@ -307,16 +307,34 @@ __device__ __forceinline__ bool mt_attempt(double x, double u, double d, double 
     return false;
 }
 
+// The two attempts of Philox block 0 (see gamma_draw): the Box-Muller pair and
+// its two acceptance uniforms.  They do not depend on the shape, so a caller
+// can form them while the shape's inputs are still being loaded.
+struct GammaFirst {
+    double x0, x1, u0, u1;
+};
+
+__device__ __forceinline__ GammaFirst gamma_first(uint32_t seed, uint32_t chain, uint32_t c0,
+                                                  uint32_t c1, uint32_t c2, uint32_t phase)
+{
+    const u32x4 w = philox4x32_10(c0, c1, c2, phase << 24, seed, chain);
+    const double rad = sqrt(-2.0 * log_pos(uo(w.x)));
+    double sn, cs;
+    sincos_d(6.283185307179586 * uo(w.y), &sn, &cs);
+    return {rad * cs, rad * sn, uo(w.z), uo(w.w)};
+}
+
 // Gamma(a, 1), Marsaglia & Tsang (ACM TOMS 26(3), 2000); a < 1 through
 // Gamma(a+1) * U^(1/a).  Philox block j (attempt field j) serves two
 // attempts: words 0,1 give a Box-Muller pair, the cos normal tried with the
 // uniform of word 2, then the sin normal with the uniform of word 3; the boost
 // uniform for a < 1 is word 0 of block kGammaBoostBlock.  FP64 throughout:
 // with shapes in the thousands (allele / ancestry counts) the acceptance test
-// d*(1 - v + ln v) cancels catastrophically in FP32.
-__device__ __forceinline__ double gamma_draw(double a, uint32_t seed, uint32_t chain,
-                                             uint32_t c0, uint32_t c1, uint32_t c2,
-                                             uint32_t phase)
+// d*(1 - v + ln v) cancels catastrophically in FP32.  Block 0's attempts come
+// in f (gamma_first of the same counters), the later blocks are drawn here.
+__device__ __forceinline__ double gamma_draw_from(double a, const GammaFirst &f, uint32_t seed,
+                                                  uint32_t chain, uint32_t c0, uint32_t c1,
+                                                  uint32_t c2, uint32_t phase)
 {
     if (!(a > 0.0)) {
         return 0.0;
@@ -325,8 +343,8 @@ __device__ __forceinline__ double gamma_draw(double a, uint32_t seed, uint32_t c
     const double d = aa - 1.0 / 3.0;
     const double c = 1.0 / sqrt(9.0 * d);
     double g = 0.0;
-    bool done = false;
-    for (uint32_t j = 0; j < kGammaMaxBlocks && !done; j++) {
+    bool done = mt_attempt(f.x0, f.u0, d, c, &g) || mt_attempt(f.x1, f.u1, d, c, &g);
+    for (uint32_t j = 1; j < kGammaMaxBlocks && !done; j++) {
         const u32x4 w = philox4x32_10(c0, c1, c2, (phase << 24) | j, seed, chain);
         const double rad = sqrt(-2.0 * log_pos(uo(w.x)));
         double sn, cs;
@@ -341,6 +359,17 @@ __device__ __forceinline__ double gamma_draw(double a, uint32_t seed, uint32_t c
         g = g * exp(log_pos(uo(b.x)) / a);
     }
     return g;
+}
+
+__device__ __forceinline__ double gamma_draw(double a, uint32_t seed, uint32_t chain,
+                                             uint32_t c0, uint32_t c1, uint32_t c2,
+                                             uint32_t phase)
+{
+    if (!(a > 0.0)) {
+        return 0.0;
+    }
+    return gamma_draw_from(a, gamma_first(seed, chain, c0, c1, c2, phase), seed, chain, c0, c1, c2,
+                           phase);
 }
 
 // Term-by-term Lanczos sum in the host routine's order (mymath.c:382-429):

@@ -16,6 +16,9 @@
 namespace sgpu {
 
 constexpr int kNT = 256;   // threads per block for the update kernels
+// UpdateP sums up to this many allele-count partials (individual tiles) from
+// straight-line loads, all in flight at once; more go through a loop.
+constexpr int kPDirectParts = 16;
 
 __device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint32_t rep,
                                    const double *parts, int nparts);
@@ -195,40 +198,57 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(5))) void k
         l = live ? idx / m.K : 0;
         k = live ? idx - l * m.K : 0;
     }
+    // Every load below is independent of the others and of nall[l]: a thread
+    // that is not live reads element 0, and a live one's (l, k, aa) lies inside
+    // the L*K*A arrays even when aa >= nall[l]; so the thread issues them all at
+    // once and forms the first gamma attempts (Philox block 0 and its
+    // Box-Muller pair, which depend on none of them) while they are in flight.
     const int na = live ? nall[l] : 0;
     const bool act = aa < na;
     const size_t pos = ((size_t)l * m.K + k) * A + aa;
-    double x = 0.0;
-    if (act) {
-        // the partials are independent loads: eight in flight, exact integer sums
-        const size_t stride = (size_t)m.L * m.K * A;
-        const int n_it = C.n_it;
+    const size_t lpos = live ? pos : 0;
+    const size_t stride = (size_t)m.L * m.K * A;
+    const int n_it = C.n_it;
+    const double Fk = F[live ? k : 0];
+    const double pri = m.freqscorr ? eps[live ? (size_t)l * A + aa : 0] : lambda[live ? k : 0];
+    int n = 0;
+    GammaFirst gf;
+    if (n_it <= kPDirectParts) {
+        // straight-line loads of every partial (indices past the last repeat it)
+        uint32_t v[kPDirectParts];
+#pragma unroll
+        for (int u = 0; u < kPDirectParts; u++) {
+            v[u] = nApart[(size_t)min(u, max(n_it - 1, 0)) * stride + lpos];
+        }
+        gf = gamma_first(m.seed, m.chain, rep, (uint32_t)l, ((uint32_t)k << 16) | (uint32_t)aa,
+                         PH_P);
+#pragma unroll
+        for (int u = 0; u < kPDirectParts; u++) {
+            n += u < n_it ? (int)v[u] : 0;
+        }
+    } else {
+        // eight in flight, exact integer sums
+        gf = gamma_first(m.seed, m.chain, rep, (uint32_t)l, ((uint32_t)k << 16) | (uint32_t)aa,
+                         PH_P);
         int n0 = 0, n1 = 0, n2 = 0, n3 = 0;
         int it = 0;
         for (; it + 8 <= n_it; it += 8) {
-            const SGPU_G uint16_t *q = nApart + it * stride + pos;
+            const SGPU_G uint16_t *q = nApart + it * stride + lpos;
             n0 += q[0] + q[4 * stride];
             n1 += q[stride] + q[5 * stride];
             n2 += q[2 * stride] + q[6 * stride];
             n3 += q[3 * stride] + q[7 * stride];
         }
-        if (it + 4 <= n_it) {
-            const SGPU_G uint16_t *q = nApart + it * stride + pos;
-            n0 += q[0];
-            n1 += q[stride];
-            n2 += q[2 * stride];
-            n3 += q[3 * stride];
-            it += 4;
-        }
         for (; it < n_it; it++) {
-            n0 += nApart[it * stride + pos];
+            n0 += nApart[it * stride + lpos];
         }
-        const int n = (n0 + n1) + (n2 + n3);
-        const double Fk = F[k];
-        const double par = m.freqscorr ? eps[(size_t)l * A + aa] * (1.0 - Fk) / Fk + (double)n
-                                       : lambda[k] + (double)n;
-        x = gamma_draw(par, m.seed, m.chain, rep, (uint32_t)l, ((uint32_t)k << 16) | (uint32_t)aa,
-                       PH_P);
+        n = (n0 + n1) + (n2 + n3);
+    }
+    double x = 0.0;
+    if (act) {
+        const double par = m.freqscorr ? pri * (1.0 - Fk) / Fk + (double)n : pri + (double)n;
+        x = gamma_draw_from(par, gf, m.seed, m.chain, rep, (uint32_t)l,
+                            ((uint32_t)k << 16) | (uint32_t)aa, PH_P);
     }
     s_x[threadIdx.x] = x;
     __syncthreads();
