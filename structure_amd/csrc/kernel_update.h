@@ -1173,37 +1173,49 @@ __device__ void eps_launch_block(const ChainDesc *__restrict__ cds, const BlockR
     const int j = threadIdx.x / K, k = threadIdx.x % K;
     const int l = bx * lpb + j;
     const bool live = j < lpb && l < m.L;
-    const int na = live ? nall[l] : 0;
+    const int na_l = nall[live ? l : 0];   // loaded with the pairs below, no branch
     const bool fst = (flags & kEpsFst) != 0;   // block-uniform
     SGPU_G double *e = eps + (size_t)(live ? l : 0) * m.A;
+    const SGPU_G double *pk = P + ((size_t)(live ? l : 0) * K + (live ? k : 0)) * m.A;
+    // two-allele data: both alleles' eps and frequencies are loaded with the
+    // locus count, before the draw picks which is a1 (16-byte aligned pairs)
+    const bool two = m.A == 2;   // uniform
+    double2 ev{0.0, 0.0}, pv{0.0, 0.0};
+    if (two) {
+        ev = *(const SGPU_G double2 *)e;
+        pv = *(const SGPU_G double2 *)pk;
+    }
+    const double Fk = F[live ? k : 0];
+    // the locus's move uniforms do not depend on the loads: formed while they
+    // are in flight (only a locus with two or more alleles uses them)
+    const u32x4 w = philox4x32_10(rep, (uint32_t)l, 0u, PH_EPS << 24, m.seed, m.chain);
+    const int na = live ? na_l : 0;
     int a1 = 0, a2 = 0;
     double d = 0.0, e1 = 0.0, e2 = 0.0;
     bool inrange = false;
-    u32x4 w{0u, 0u, 0u, 0u};
     if (fst && (int)threadIdx.x < (m.onefst ? 1 : K)) {
         s_newf[threadIdx.x] = fst_proposal(m, F[threadIdx.x], threadIdx.x, rep, nullptr);
     }
     if (na > 1) {
-        w = philox4x32_10(rep, (uint32_t)l, 0u, PH_EPS << 24, m.seed, m.chain);
         a1 = (int)(uz(w.x) * (double)na);
         a2 = (int)(uz(w.y) * (double)(na - 1));
         if (a2 >= a1) {
             a2 += 1;
         }
         d = 0.0 + uz(w.z) * (m.invsqrtn - 0.0);
-        e1 = e[a1];
-        e2 = e[a2];
+        e1 = two ? (a1 ? ev.y : ev.x) : e[a1];
+        e2 = two ? (a2 ? ev.y : ev.x) : e[a2];
         inrange = (e1 + d < 1.0) && (e2 - d > 0.0);
     }
-    const double frac = (1.0 - F[k]) / F[k];
+    const double frac = (1.0 - Fk) / Fk;
     double lg0 = 0.0, lg1 = 0.0, lg2 = 0.0, lg3 = 0.0, lp1 = 0.0, lp2 = 0.0, tk = 0.0;
     if (inrange) {
         lg0 = lgamma_lanczos(frac * e1);
         lg1 = lgamma_lanczos(frac * e2);
         lg2 = lgamma_lanczos(frac * (e1 + d));
         lg3 = lgamma_lanczos(frac * (e2 - d));
-        lp1 = log_d(P[((size_t)l * K + k) * m.A + a1]);
-        lp2 = log_d(P[((size_t)l * K + k) * m.A + a2]);
+        lp1 = log_d(two ? (a1 ? pv.y : pv.x) : pk[a1]);
+        lp2 = log_d(two ? (a2 ? pv.y : pv.x) : pk[a2]);
         // pop k's six terms in the reference's order within a pop
         tk = ((((lg0 + lg1) - lg2) - lg3) + frac * d * lp1) - frac * d * lp2;
     }
