@@ -142,6 +142,30 @@ __global__ __launch_bounds__(kNT) void k_lambda_partial(const ChainDesc *__restr
     lambda_partial_block(C, m, bx);
 }
 
+// UpdateP's allele count of one (locus, pop, allele): the sum of its n_it
+// individual-tile partials (plane stride `stride`), read by NP >= n_it
+// straight-line loads all in flight at once (loads past the last partial
+// repeat it and are not added); the first gamma attempts of the thread's
+// Philox counters are formed while the loads are outstanding.
+template <int NP>
+__device__ __forceinline__ int part_sum_overlapped(const SGPU_G uint16_t *q, size_t stride, int n_it,
+                                                   GammaFirst &gf, uint32_t seed, uint32_t chain,
+                                                   uint32_t rep, uint32_t l, uint32_t c2)
+{
+    uint32_t v[NP];
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+        v[u] = q[(size_t)min(u, n_it - 1) * stride];
+    }
+    gf = gamma_first(seed, chain, rep, l, c2, PH_P);
+    int n = 0;
+#pragma unroll
+    for (int u = 0; u < NP; u++) {
+        n += u < n_it ? (int)v[u] : 0;
+    }
+    return n;
+}
+
 // ---------------------------------------------------------------------------
 // UpdateP (UpdateP.c:106-199): one thread per (locus, pop, allele), a block
 // holding whole (locus, pop) groups: nall[l] threads each from the chain's work
@@ -211,44 +235,38 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(5))) void k
     const int n_it = C.n_it;
     const double Fk = F[live ? k : 0];
     const double pri = m.freqscorr ? eps[live ? (size_t)l * A + aa : 0] : lambda[live ? k : 0];
+    const uint32_t c2 = ((uint32_t)k << 16) | (uint32_t)aa;
+    const SGPU_G uint16_t *q = nApart + lpos;
     int n = 0;
     GammaFirst gf;
-    if (n_it <= kPDirectParts) {
-        // straight-line loads of every partial (indices past the last repeat it)
-        uint32_t v[kPDirectParts];
-#pragma unroll
-        for (int u = 0; u < kPDirectParts; u++) {
-            v[u] = nApart[(size_t)min(u, max(n_it - 1, 0)) * stride + lpos];
-        }
-        gf = gamma_first(m.seed, m.chain, rep, (uint32_t)l, ((uint32_t)k << 16) | (uint32_t)aa,
-                         PH_P);
-#pragma unroll
-        for (int u = 0; u < kPDirectParts; u++) {
-            n += u < n_it ? (int)v[u] : 0;
-        }
+    if (n_it <= 4) {
+        n = part_sum_overlapped<4>(q, stride, n_it, gf, m.seed, m.chain, rep, (uint32_t)l, c2);
+    } else if (n_it <= 8) {
+        n = part_sum_overlapped<8>(q, stride, n_it, gf, m.seed, m.chain, rep, (uint32_t)l, c2);
+    } else if (n_it <= kPDirectParts) {
+        n = part_sum_overlapped<kPDirectParts>(q, stride, n_it, gf, m.seed, m.chain, rep,
+                                               (uint32_t)l, c2);
     } else {
         // eight in flight, exact integer sums
-        gf = gamma_first(m.seed, m.chain, rep, (uint32_t)l, ((uint32_t)k << 16) | (uint32_t)aa,
-                         PH_P);
+        gf = gamma_first(m.seed, m.chain, rep, (uint32_t)l, c2, PH_P);
         int n0 = 0, n1 = 0, n2 = 0, n3 = 0;
         int it = 0;
         for (; it + 8 <= n_it; it += 8) {
-            const SGPU_G uint16_t *q = nApart + it * stride + lpos;
-            n0 += q[0] + q[4 * stride];
-            n1 += q[stride] + q[5 * stride];
-            n2 += q[2 * stride] + q[6 * stride];
-            n3 += q[3 * stride] + q[7 * stride];
+            const SGPU_G uint16_t *qi = q + it * stride;
+            n0 += qi[0] + qi[4 * stride];
+            n1 += qi[stride] + qi[5 * stride];
+            n2 += qi[2 * stride] + qi[6 * stride];
+            n3 += qi[3 * stride] + qi[7 * stride];
         }
         for (; it < n_it; it++) {
-            n0 += nApart[it * stride + lpos];
+            n0 += q[it * stride];
         }
         n = (n0 + n1) + (n2 + n3);
     }
     double x = 0.0;
     if (act) {
         const double par = m.freqscorr ? pri * (1.0 - Fk) / Fk + (double)n : pri + (double)n;
-        x = gamma_draw_from(par, gf, m.seed, m.chain, rep, (uint32_t)l,
-                            ((uint32_t)k << 16) | (uint32_t)aa, PH_P);
+        x = gamma_draw_from(par, gf, m.seed, m.chain, rep, (uint32_t)l, c2, PH_P);
     }
     s_x[threadIdx.x] = x;
     __syncthreads();
