@@ -82,6 +82,35 @@ def test_update_z_counts_and_like_bitexact(kind, K):
         _close(like, och.calc_like(Q, P))
 
 
+@pytest.mark.parametrize("N", [30, 100, 200, 300])
+def test_update_p_sums_every_tile_partial(N):
+    """UpdateP on the per-tile allele-count partials that UpdateZ itself leaves
+    (a host write of the counts fills only the first tile).  With few loci each
+    16-individual word row is a tile of its own, so N = 30, 100, 200 and 300
+    give 2, 7, 13 and 19 partials: the straight-line sums of 4, 8 and 16 loads
+    and the loop beyond 16.  Two-allele and F-model parameters, P against the
+    oracle's UpdateP on the oracle's counts."""
+    rng = np.random.default_rng(3000 + N)
+    geno, nall, A = _snp(N, 300, 3, seed=N)
+    K = 3
+    och, gch = _pair(geno, nall, K, A, fmt=sa.GENO_SNP2, seed=31, chain=1, freqscorr=1)
+    gch.init()
+    och.init()
+    Q, P = _state(gch, nall, rng, K, A, fq=0.6)
+    eps = och.eps.copy()
+    F = rng.uniform(0.01, 0.4, K)
+    gch.set(sa.FST, F)
+    lam = np.ones(K)
+    for rep in (4, 5):
+        gch.update_z(rep, like=False)
+        nA, _ = och.update_z(Q, P, rep)
+        assert np.array_equal(gch.get(sa.NA), nA)
+        gch.update_p(rep, collect=False)
+        P = och.update_p(nA, eps, F, lam, rep)
+        _close(gch.get(sa.P), P)
+        gch.set(sa.P, P)
+
+
 @pytest.mark.parametrize("K", list(range(1, 21)))
 def test_burnin_update_z_counts_bitexact(K):
     """UpdateZ without the lnP terms (burn-in sweeps), 2-bit words, every K up to
