@@ -136,7 +136,7 @@ int sgpu_create(const sgpu_model *m, const sgpu_data *d, sgpu_ctx **out);
  *   z_blocks      blocks per 2-bit UpdateZ launch over the whole batch (sets the
  *                 individuals per block; default 4096)
  *   metro_blocks  blocks per Metropolis-ratio launch (sets the loci per block;
- *                 default 1024)
+ *                 default 2048)
  *   byte_loci     loci per byte-format UpdateZ block (default: the longest of 64,
  *                 32, 16 that gives z_blocks blocks over the batch, else 16 to 64
  *                 by K; at most 256)
