@@ -255,11 +255,23 @@ def test_update_lambda_matches_oracle(freqscorr, popspec):
     assert 2 < moved < 30
 
 
+@pytest.mark.parametrize("kind", ["microsat", "biallelic"])
 @pytest.mark.parametrize("lam0", [1.0, 0.6])
-def test_update_eps_matches_oracle(lam0):
+def test_update_eps_matches_oracle(lam0, kind):
+    """Both the general path and the two-allele one (A = 2: both alleles' eps and
+    frequencies loaded before the draw picks a1), with monomorphic and empty
+    loci among the two-allele ones."""
     rng = np.random.default_rng(11)
-    d = sa.synth_microsat(25, 400, amin=1, amax=7, Ktrue=3, seed=10)
-    geno, nall, A = d["geno"], d["nall"], d["A"]
+    if kind == "microsat":
+        d = sa.synth_microsat(25, 400, amin=1, amax=7, Ktrue=3, seed=10)
+        geno, nall, A = d["geno"], d["nall"], d["A"]
+    else:
+        geno, nall = random_geno(rng, 25, 400, 2, miss=0.05)
+        nall[::7] = 1
+        geno[:, :, ::7] = np.where(geno[:, :, ::7] >= 0, 0, -9)
+        nall[::11] = 0
+        geno[:, :, ::11] = -9
+        A = 2
     K = 3
     och, gch = _pair(geno, nall, K, A, seed=14)
     gch.init()
