@@ -25,6 +25,7 @@ __device__ void lambda_final_block(const ChainDesc &C, const ModelArgs &m, uint3
 template <int NF>
 __device__ void gibbs_q_block(const ChainDesc &C, const ModelArgs &m, int bx, uint32_t rep,
                               int collect);
+__device__ void metro_propose_block(const ChainDesc &C, const ModelArgs &m, int bx, uint32_t rep);
 
 // The chain and block coordinates of this block (uniform: scalar loads).
 #define SGPU_BLOCK_PROLOGUE()                          \
@@ -194,6 +195,10 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(5))) void k
     }
     if (br_.role == kRoleQGibbs) {
         gibbs_q_block<8>(C, m, bx, rep, collect);   // few loads in flight: UpdateP's registers
+        return;
+    }
+    if (br_.role == kRoleMetroProp) {
+        metro_propose_block(C, m, bx, rep);
         return;
     }
     const SGPU_G int32_t *nall = g(C.nall);
@@ -560,6 +565,14 @@ __global__ __launch_bounds__(kNT) void k_metro_propose(const ChainDesc *__restri
         lambda_partial_block(C, m, bx);
         return;
     }
+    metro_propose_block(C, m, bx, rep);
+}
+
+// One block of the Metropolis proposals TestQ ~ Dirichlet(alpha) (UpdateQMetro,
+// UpdateQ.c:199-307): whole individuals, one thread per (individual, pop).  They
+// read only alpha, so in a sweep they run as blocks of the UpdateP launch.
+__device__ void metro_propose_block(const ChainDesc &C, const ModelArgs &m, int bx, uint32_t rep)
+{
     __shared__ double s_x[kNT];
     const int K = m.K;
     const int ipb = kNT / K;   // one thread per (individual, pop), whole individuals per block

@@ -136,12 +136,13 @@ struct sgpu_ctx {
         p_lam_fin, p_eps, p_fst_part, p_fst_fin, p_collect, p_init_eps;
     // the fused launches of a sweep: UpdateP with the lambda decision blocks of
     // F-model chains, in Gibbs sweeps also with the Gibbs Q blocks
-    // (p_update_pq_sw); in Metropolis sweeps the proposal launch with the lambda
-    // log-sum blocks of chains without the F model, which in Gibbs sweeps run
-    // in a launch of their own (p_lam_part_p); the epsilon launch with the scalar
+    // (p_update_pq_sw), in Metropolis sweeps with the Q proposal blocks
+    // (p_update_pm_sw); in Metropolis sweeps the proposal launch keeps only the
+    // lambda log-sum blocks of chains without the F model (none: no launch), which
+    // in Gibbs sweeps run in a launch of their own (p_lam_part_p); the epsilon launch with the scalar
     // blocks, epsilon + Fst blocks and count-fold blocks; the Fst decision
     // (F-model chains); separate lambda launches for pop-specific lambda with F
-    Plan p_update_p_sw, p_update_pq_sw, p_propose_sw, p_lam_part_p, p_postz, p_tail,
+    Plan p_update_pq_sw, p_update_pm_sw, p_propose_sw, p_lam_part_p, p_postz, p_tail,
         p_lam_part_x, p_lam_fin_x;
     Plan p_eps_logsum;   // the log-eps sums of F-model lambda chains, outside a sweep
     // nQfold holds the fold of the current nQpart (false after UpdateZ or a host
@@ -1033,19 +1034,19 @@ int build_plans(sgpu_ctx *c)
         auto p_dims = [](const HostChain &h) { return std::make_pair(h.d.p_blocks, 1); };
         std::vector<BlockRef> v;
         add_blocks(c, v, mode(kLamEps), one, kRoleLambda);
-        add_blocks(c, v, all_chains, p_dims);
-        if ((rc = upload_plan(c, c->p_update_p_sw, v, kXFast))) {
-            return rc;
-        }
-        v.clear();
-        add_blocks(c, v, mode(kLamEps), one, kRoleLambda);
         add_blocks(c, v, all_chains, gibbs_dims, kRoleQGibbs);
         add_blocks(c, v, all_chains, p_dims);
         if ((rc = upload_plan(c, c->p_update_pq_sw, v, kXFast))) {
             return rc;
         }
         v.clear();
-        add_blocks(c, v, all_chains, gibbs_dims);
+        add_blocks(c, v, mode(kLamEps), one, kRoleLambda);
+        add_blocks(c, v, all_chains, gibbs_dims, kRoleMetroProp);
+        add_blocks(c, v, all_chains, p_dims);
+        if ((rc = upload_plan(c, c->p_update_pm_sw, v, kXFast))) {
+            return rc;
+        }
+        v.clear();
         add_blocks(c, v, mode(kLamP), lb, kRoleLamPart);
         if ((rc = upload_plan(c, c->p_propose_sw, v, kXFast))) {
             return rc;
@@ -1509,7 +1510,7 @@ int sgpu_set_state(sgpu_ctx *c, int32_t which, const void *host, size_t bytes)
 
 namespace {
 
-// UpdateP over the plan pp: p_update_p, or a sweep's p_update_p_sw / p_update_pq_sw
+// UpdateP over the plan pp: p_update_p, or a sweep's p_update_pm_sw / p_update_pq_sw
 // with lambda decision and Gibbs Q blocks.
 int update_p(sgpu_ctx *c, uint32_t rep, int32_t collect, const Plan &pp)
 {
@@ -1565,7 +1566,9 @@ int update_q(sgpu_ctx *c, uint32_t rep, int32_t collect, const Plan &qp)
     const uint32_t *r = c->repd;
     const bool metro = c->metrofreq > 0 && rep % (uint32_t)c->metrofreq == 0;
     if (metro) {
-        if ((rc = launch(c, k_metro_propose, qp, kNT, r))) {   // (individual, pop) grid
+        // (individual, pop) grid; in a sweep the proposals ran in the UpdateP
+        // launch and qp holds only lambda log-sum blocks, if any
+        if (qp.nblocks > 0 && (rc = launch(c, k_metro_propose, qp, kNT, r))) {
             return rc;
         }
         for (const KGroup &g : c->metro_groups) {
@@ -1726,7 +1729,7 @@ int sgpu_sweep(sgpu_ctx *c, uint32_t rep)
     const bool metro = c->metrofreq > 0 && rep % (uint32_t)c->metrofreq == 0;
     if ((rc = ensure_lam(c))) return rc;
     if (metro) {
-        if ((rc = update_p(c, rep, collect, c->p_update_p_sw))) return rc;
+        if ((rc = update_p(c, rep, collect, c->p_update_pm_sw))) return rc;
         if ((rc = update_q(c, rep, collect, c->p_propose_sw))) return rc;
     } else {
         if ((rc = ensure_fold(c))) return rc;

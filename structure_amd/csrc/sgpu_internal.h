@@ -122,7 +122,8 @@ enum : uint32_t {
     kRoleFold = 2,      // epsilon launch: one block of the ancestry-count fold
     kRoleScalars = 3,   // epsilon launch: alpha, the lnP(D) sum and the scalar posterior sums
     kRoleLamPart = 4,   // Metropolis proposal launch: 256 loci of the lambda log sums of P
-    kRoleQGibbs = 5     // UpdateP launch of a Gibbs sweep: a block of the Gibbs Q update
+    kRoleQGibbs = 5,    // UpdateP launch of a Gibbs sweep: a block of the Gibbs Q update
+    kRoleMetroProp = 6  // UpdateP launch of a Metropolis sweep: a block of the Q proposals
 };
 
 // Where a sweep runs a chain's lambda update (UpdateLambda.c:8-102): its log
